@@ -1,0 +1,603 @@
+// api.hip -- the C ABI of libdakc_amd (include/dakc.h): contexts, job
+// planning, device workspaces and the launch sequence of one count.
+//
+// Replaces the reference's kmercounter constructor + perform_kcount
+// (kcounter.hpp:84-99, kcounter.cpp:247-414): the record buffer goes in,
+// every stage runs on the GPU, the sorted (kmer, count) table comes out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/dakc.h"
+#include "dk_api_internal.hpp"
+#include "dk_gen.hpp"
+#include "dk_kernels.hpp"
+
+using namespace dk;
+
+// ------------------------------------------------------------------ status --
+
+extern "C" const char *dakc_status_str(int s) {
+  switch (s) {
+  case DAKC_OK: return "ok";
+  case DAKC_E_INVAL: return "invalid argument";
+  case DAKC_E_FORMAT: return "malformed read buffer (records must be read_len bytes + '\\n')";
+  case DAKC_E_NOMEM: return "out of memory";
+  case DAKC_E_TOOBIG: return "job has 2^32 or more k-mer slots";
+  case DAKC_E_DEVICE: return "HIP runtime error";
+  case DAKC_E_NODEVICE: return "no usable GPU";
+  case DAKC_E_REJECTED: return "job exceeds the served-mode HBM budget";
+  case DAKC_E_NOTFOUND: return "unknown ticket";
+  case DAKC_E_PENDING: return "job still running";
+  case DAKC_E_CANCELLED: return "job cancelled";
+  default: return "unknown status";
+  }
+}
+
+extern "C" int dakc_abi_version(void) { return DAKC_ABI_VERSION; }
+
+namespace dk {
+
+// ------------------------------------------------------------ workspaces --
+
+int DevBuf::ensure(size_t bytes, hipStream_t st, bool zero) {
+  if (bytes <= cap) return DAKC_OK;
+  if (p) {
+    hipStreamSynchronize(st);
+    hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  size_t want = (bytes + 4095) & ~(size_t)4095;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    return DAKC_E_NOMEM;
+  }
+  cap = want;
+  if (zero && hipMemsetAsync(p, 0, want, st) != hipSuccess) return DAKC_E_DEVICE;
+  return DAKC_OK;
+}
+
+void DevBuf::release() {
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+JobWs::~JobWs() {
+  for (auto *b : {&hist, &dstart, &ndev, &err, &tickets, &flags, &rle_flags, &keys_a, &keys_b,
+                  &counts, &binoff, &reads})
+    b->release();
+  for (auto &e : ev)
+    if (e) hipEventDestroy(e);
+}
+
+int JobWs::init(hipStream_t st) {
+  stream = st;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return DAKC_E_DEVICE;
+  return DAKC_OK;
+}
+
+uint32_t JobWs::next_epoch() {
+  if (epoch + 1 >= (1u << 14)) {
+    launch_fill_u64((uint64_t *)flags.p, 0, flags.cap / 8, stream);
+    launch_fill_u64((uint64_t *)rle_flags.p, 0, rle_flags.cap / 8, stream);
+    epoch = 0;
+  }
+  return ++epoch;
+}
+
+Workspace JobWs::view() const {
+  Workspace w;
+  w.hist = (uint64_t *)hist.p;
+  w.digit_start = (uint64_t *)dstart.p;
+  w.n_dev = (uint64_t *)ndev.p;
+  w.err = (uint32_t *)err.p;
+  w.tickets = (uint32_t *)tickets.p;
+  w.flags = (uint64_t *)flags.p;
+  w.rle_flags = (uint64_t *)rle_flags.p;
+  w.keys_a = keys_a.p;
+  w.keys_b = keys_b.p;
+  w.counts = (uint32_t *)counts.p;
+  w.bin_offsets = (uint64_t *)binoff.p;
+  return w;
+}
+
+size_t job_device_bytes(const Plan &pl) {
+  const size_t kb = (size_t)pl.kw * 8;
+  size_t b = 0;
+  b += 2 * (size_t)MAX_PASSES * 256 * 8 + 64 + 4096;
+  b += (size_t)std::max<uint64_t>(pl.tiles0, pl.tiles_keys) * 256 * 8 + (size_t)pl.tiles_keys * 8;
+  b += 2 * (size_t)pl.nmax * kb + (size_t)pl.nmax * 4;
+  b += (size_t)pl.nbytes + 256;
+  return b;
+}
+
+int JobWs::reserve(const Plan &pl, bool need_reads) {
+  int s;
+  const size_t kb = (size_t)pl.kw * 8;
+  if ((s = hist.ensure((size_t)MAX_PASSES * 256 * 8, stream, false))) return s;
+  if ((s = dstart.ensure((size_t)MAX_PASSES * 256 * 8, stream, false))) return s;
+  if ((s = ndev.ensure(64, stream, false))) return s;
+  if ((s = err.ensure(64, stream, false))) return s;
+  if ((s = tickets.ensure((MAX_PASSES + 2) * 4, stream, false))) return s;
+  const size_t ft = std::max<uint64_t>(pl.tiles0, pl.tiles_keys);
+  const size_t fl_before = flags.cap, rf_before = rle_flags.cap;
+  if ((s = flags.ensure(std::max<size_t>(ft, 1) * 256 * 8, stream, true))) return s;
+  if ((s = rle_flags.ensure(std::max<size_t>(pl.tiles_keys, 1) * 8, stream, true))) return s;
+  if (flags.cap != fl_before || rf_before != rle_flags.cap) {
+    // freshly zeroed arrays: restart epochs for both so no stale word matches
+    launch_fill_u64((uint64_t *)flags.p, 0, flags.cap / 8, stream);
+    launch_fill_u64((uint64_t *)rle_flags.p, 0, rle_flags.cap / 8, stream);
+    epoch = 0;
+  }
+  if ((s = keys_a.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
+  if ((s = keys_b.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
+  if ((s = counts.ensure(std::max<size_t>(pl.nmax, 1) * 4 + 256, stream, false))) return s;
+  if ((s = binoff.ensure(((size_t)pl.num_pes + 1) * 8 + 64, stream, false))) return s;
+  if (need_reads && (s = reads.ensure(pl.nbytes + 256, stream, false))) return s;
+  return DAKC_OK;
+}
+
+// ------------------------------------------------------------- planning --
+
+int make_plan(const dakc_job *job, Plan *pl) {
+  if (!job || !pl) return DAKC_E_INVAL;
+  if (job->format != DAKC_FMT_FIXED) return DAKC_E_INVAL;
+  if (job->k < 1 || job->k > 64) return DAKC_E_INVAL;
+  if (job->read_len < 1 || job->read_len > MAX_READ_LEN) return DAKC_E_INVAL;
+  if (job->num_pes < 1) return DAKC_E_INVAL;
+  if (job->owner_bin >= 0 && (uint32_t)job->owner_bin >= job->num_pes) return DAKC_E_INVAL;
+  if (job->owner_bin < -1) return DAKC_E_INVAL;
+  if (job->num_pes > 256 && job->owner_bin == -1) return DAKC_E_INVAL;  // one owner digit
+  if (job->nbytes && !job->reads) return DAKC_E_INVAL;
+  if (job->reads_on_device && ((uintptr_t)job->reads & 3)) return DAKC_E_INVAL;
+  Plan &p = *pl;
+  p = Plan();
+  p.kw = job->k <= 32 ? 1 : 2;
+  p.k = job->k;
+  p.L = (uint32_t)job->read_len;
+  p.stride = p.L + 1;
+  p.nbytes = job->nbytes;
+  p.num_pes = job->num_pes;
+  p.owner_bin = job->num_pes > 1 ? job->owner_bin : -1;
+  const uint64_t rem = p.nbytes % p.stride;
+  p.nrec = p.nbytes / p.stride;
+  p.last_has_nl = 1;
+  if (rem == p.L) {
+    p.nrec += 1;
+    p.last_has_nl = 0;
+  } else if (rem != 0) {
+    return DAKC_E_FORMAT;
+  }
+  p.S = p.L >= (uint32_t)p.k ? p.L - p.k + 1 : 0;
+  const unsigned __int128 nm = (unsigned __int128)p.nrec * p.S;
+  if (nm >= ((unsigned __int128)1 << 32)) return DAKC_E_TOOBIG;
+  p.nmax = (uint64_t)nm;
+  const int tile = tile_keys(p.kw);
+  p.tile = tile;
+  const uint32_t wpr = (p.L + 31) / 32 + 2;
+  uint32_t rt = p.S ? tile / p.S : tile;
+  rt = std::min<uint32_t>(rt, 128);
+  rt = std::min<uint32_t>(rt, RAW_CAP / p.stride);
+  rt = std::min<uint32_t>(rt, 896 / wpr);
+  rt = std::max<uint32_t>(rt, 1);
+  p.RT = rt;
+  p.wpr = wpr;
+  p.tiles0 = (uint32_t)((p.nrec + rt - 1) / rt);
+  p.tiles_keys = (uint32_t)((p.nmax + tile - 1) / tile);
+  p.ndig = (2 * p.k + 7) / 8;
+  p.owner_digit = (p.num_pes > 1 && p.owner_bin < 0) ? 1 : 0;
+  p.npasses = p.ndig + p.owner_digit;
+  p.canonical = job->canonical ? 1 : 0;
+  return DAKC_OK;
+}
+
+ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
+  ReadsDesc rd;
+  rd.reads = dreads;
+  rd.nbytes = p.nbytes;
+  rd.nrec = p.nrec;
+  rd.L = p.L;
+  rd.stride = p.stride;
+  rd.k = (uint32_t)p.k;
+  rd.S = p.S;
+  rd.RT = p.RT;
+  rd.wpr = p.wpr;
+  rd.invS = p.S ? 1.0f / (float)p.S : 0.0f;
+  rd.canonical = p.canonical;
+  rd.num_pes = p.num_pes;
+  rd.owner_bin = p.owner_bin;
+  rd.last_has_nl = p.last_has_nl;
+  return rd;
+}
+
+// Enqueue the whole count of one job on ws.stream.  The sorted table ends in
+// *table_keys (one of the two key buffers) and ws.counts.
+int enqueue_count(JobWs &ws, const Plan &pl, const uint8_t *dreads, int num_cus, bool events,
+                  void **table_keys) {
+  hipStream_t st = ws.stream;
+  Workspace w = ws.view();
+  const ReadsDesc rd = plan_reads(pl, dreads);
+  launch_fill_u64(w.hist, 0, (uint64_t)pl.npasses * 256, st);
+  launch_fill_u64(w.n_dev, 0, 2, st);
+  launch_fill_u32(w.err, 0, 1, st);
+  launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
+  if (events) hipEventRecord(ws.ev[0], st);
+  if (pl.nrec > 0) {
+    const int grid = (int)std::min<uint64_t>(pl.tiles0, (uint64_t)num_cus * 4);
+    launch_hist_reads(pl.kw, rd, pl.ndig, pl.owner_digit, w, pl.tiles0, grid, st);
+    launch_scan_digits(pl.npasses, w, st);
+  }
+  if (events) hipEventRecord(ws.ev[1], st);
+  void *cur = w.keys_a, *other = w.keys_b;
+  if (pl.nmax > 0) {
+    for (int p = 0; p < pl.npasses; ++p) {
+      const bool owner_pass = pl.owner_digit && p == pl.npasses - 1;
+      const uint32_t ep = ws.next_epoch();
+      if (p == 0) {
+        launch_sweep(pl.kw, true, rd, nullptr, cur, 0, false, pl.num_pes, w.digit_start, w, 0, ep,
+                     pl.tiles0, st);
+      } else {
+        launch_sweep(pl.kw, false, rd, cur, other, 8 * p, owner_pass, pl.num_pes,
+                     w.digit_start + 256 * p, w, p, ep, pl.tiles_keys, st);
+        std::swap(cur, other);
+      }
+      if (events) hipEventRecord(ws.ev[2 + p], st);
+    }
+    const uint32_t ep = ws.next_epoch();
+    launch_rle(pl.kw, cur, other, w, ep, pl.tiles_keys, st);
+    if (pl.owner_digit) launch_bin_offsets(pl.kw, other, pl.num_pes, w, st);
+  }
+  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
+  *table_keys = other;
+  return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
+// After the stream has drained: read n, d and the error word.
+int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d) {
+  uint64_t nd[2] = {0, 0};
+  uint32_t e = 0;
+  if (hipMemcpy(nd, ws.ndev.p, 16, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
+  if (hipMemcpy(&e, ws.err.p, 4, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
+  if (e) return DAKC_E_FORMAT;
+  if (pl.nmax == 0) nd[1] = 0;
+  *n = nd[0];
+  *d = nd[1];
+  return DAKC_OK;
+}
+
+}  // namespace dk
+
+// --------------------------------------------------------------- context --
+
+extern "C" int dakc_ctx_create(int device, const dakc_opts *opts, dakc_ctx **out) {
+  if (!out) return DAKC_E_INVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return DAKC_E_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) return DAKC_E_INVAL;
+  if (hipSetDevice(device) != hipSuccess) return DAKC_E_DEVICE;
+  dakc_ctx *c = new (std::nothrow) dakc_ctx();
+  if (!c) return DAKC_E_NOMEM;
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return DAKC_E_DEVICE;
+  }
+  c->num_cus = prop.multiProcessorCount;
+  if (opts) c->opts = *opts;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return DAKC_E_DEVICE;
+  }
+  int s = c->ws.init(c->stream);
+  if (s) {
+    dakc_ctx_destroy(c);
+    return s;
+  }
+  *out = c;
+  return DAKC_OK;
+}
+
+extern "C" int dakc_ctx_destroy(dakc_ctx *c) {
+  if (!c) return DAKC_E_INVAL;
+  hipSetDevice(c->device);
+  serve_shutdown(c);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  c->ws.~JobWs();
+  new (&c->ws) JobWs();
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return DAKC_OK;
+}
+
+// ----------------------------------------------------------- plain count --
+
+static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool to_host) {
+  if (!c || !job || !out) return DAKC_E_INVAL;
+  std::memset(out, 0, sizeof(*out));
+  hipSetDevice(c->device);
+  Plan pl;
+  int s = make_plan(job, &pl);
+  if (s) return s;
+  JobWs &ws = c->ws;
+  if ((s = ws.reserve(pl, !job->reads_on_device))) return s;
+  const uint8_t *dreads = (const uint8_t *)job->reads;
+  if (!job->reads_on_device && pl.nbytes) {
+    if (hipMemcpyAsync(ws.reads.p, job->reads, pl.nbytes, hipMemcpyHostToDevice, ws.stream) !=
+        hipSuccess)
+      return DAKC_E_DEVICE;
+    dreads = (const uint8_t *)ws.reads.p;
+  }
+  void *tkeys = nullptr;
+  if ((s = enqueue_count(ws, pl, dreads, c->num_cus, true, &tkeys))) return s;
+  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+  uint64_t n = 0, d = 0;
+  if ((s = finish_count(ws, pl, &n, &d))) return s;
+
+  // stage timings
+  dakc_stats &st = c->last;
+  std::memset(&st, 0, sizeof(st));
+  float ms = 0;
+  st.npasses = pl.nmax ? (uint32_t)pl.npasses : 0;
+  st.tiles0 = pl.tiles0;
+  st.n = n;
+  st.d = d;
+  hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]);
+  st.hist_ms = ms;
+  if (pl.nmax) {
+    hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);
+    st.pass0_ms = ms;
+    st.sweep_ms[0] = ms;
+    for (int p = 1; p < pl.npasses; ++p) {
+      hipEventElapsedTime(&ms, ws.ev[1 + p], ws.ev[2 + p]);
+      st.sweep_ms[p] = ms;
+      st.passes_ms += ms;
+    }
+    hipEventElapsedTime(&ms, ws.ev[1 + pl.npasses], ws.ev[2 + MAX_PASSES]);
+    st.rle_ms = ms;
+  }
+  hipEventElapsedTime(&ms, ws.ev[0], ws.ev[2 + MAX_PASSES]);
+  st.total_ms = ms;
+
+  out->n = d;
+  out->total = n;
+  out->key_words = pl.kw;
+  out->k = pl.k;
+  out->num_bins = pl.owner_digit ? pl.num_pes : 0;
+  if (pl.owner_digit) {
+    out->bin_offsets = (uint64_t *)std::malloc(((size_t)pl.num_pes + 1) * 8);
+    if (!out->bin_offsets) return DAKC_E_NOMEM;
+    if (d) {
+      if (hipMemcpy(out->bin_offsets, ws.binoff.p, ((size_t)pl.num_pes + 1) * 8,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return DAKC_E_DEVICE;
+    } else {
+      std::memset(out->bin_offsets, 0, ((size_t)pl.num_pes + 1) * 8);
+    }
+  }
+  if (!to_host) {
+    out->keys = (uint64_t *)tkeys;
+    out->counts = (uint32_t *)ws.counts.p;
+    out->on_device = 1;
+    return DAKC_OK;
+  }
+  const size_t kbytes = (size_t)d * pl.kw * 8;
+  out->keys = (uint64_t *)std::malloc(kbytes ? kbytes : 8);
+  out->counts = (uint32_t *)std::malloc(d ? (size_t)d * 4 : 4);
+  if (!out->keys || !out->counts) {
+    dakc_table_free(out);
+    return DAKC_E_NOMEM;
+  }
+  if (d) {
+    if (hipMemcpy(out->keys, tkeys, kbytes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(out->counts, ws.counts.p, (size_t)d * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      dakc_table_free(out);
+      return DAKC_E_DEVICE;
+    }
+  }
+  return DAKC_OK;
+}
+
+extern "C" int dakc_count(dakc_ctx *ctx, const dakc_job *job, dakc_table *out) {
+  return count_common(ctx, job, out, true);
+}
+
+extern "C" int dakc_count_device(dakc_ctx *ctx, const dakc_job *job, dakc_table *out) {
+  return count_common(ctx, job, out, false);
+}
+
+extern "C" int dakc_last_stats(dakc_ctx *ctx, dakc_stats *out) {
+  if (!ctx || !out) return DAKC_E_INVAL;
+  *out = ctx->last;
+  return DAKC_OK;
+}
+
+extern "C" void dakc_table_free(dakc_table *t) {
+  if (!t) return;
+  if (!t->on_device) {
+    std::free(t->keys);
+    std::free(t->counts);
+  }
+  std::free(t->bin_offsets);
+  std::memset(t, 0, sizeof(*t));
+}
+
+extern "C" uint64_t dakc_table_serialize(const dakc_table *t, void *buf, uint64_t cap) {
+  if (!t || t->on_device) return 0;
+  const uint64_t rec = (uint64_t)t->key_words * 8 + 4;
+  const uint64_t need = rec * t->n;
+  if (!buf) return need;
+  if (cap < need) return 0;
+  uint8_t *o = (uint8_t *)buf;
+  for (uint64_t i = 0; i < t->n; ++i) {
+    std::memcpy(o, t->keys + i * t->key_words, (size_t)t->key_words * 8);
+    std::memcpy(o + t->key_words * 8, t->counts + i, 4);
+    o += rec;
+  }
+  return need;
+}
+
+// ----------------------------------------------------------- batched count --
+
+extern "C" int dakc_count_batch(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs,
+                                dakc_table *outs) {
+  if (!ctx || (njobs && (!jobs || !outs))) return DAKC_E_INVAL;
+  return batch_count(ctx, jobs, njobs, outs);
+}
+
+// ------------------------------------------------------- device helpers --
+
+extern "C" int dakc_device_alloc(dakc_ctx *ctx, uint64_t nbytes, void **dptr) {
+  if (!ctx || !dptr) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  if (hipMalloc(dptr, nbytes ? nbytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    return DAKC_E_NOMEM;
+  }
+  return DAKC_OK;
+}
+extern "C" int dakc_device_free(dakc_ctx *ctx, void *dptr) {
+  if (!ctx) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  return hipFree(dptr) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+extern "C" int dakc_memcpy_h2d(dakc_ctx *ctx, void *dst, const void *src, uint64_t n) {
+  if (!ctx) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+extern "C" int dakc_memcpy_d2h(dakc_ctx *ctx, void *dst, const void *src, uint64_t n) {
+  if (!ctx) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  return hipMemcpy(dst, src, n, hipMemcpyDeviceToHost) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
+// ---------------------------------------------------------- generator --
+
+namespace dk {
+
+int make_gen_core(const dakc_gen_params *p, GenCore *g) {
+  if (!p || !g) return DAKC_E_INVAL;
+  if (p->read_len < 1 || p->read_len > MAX_READ_LEN) return DAKC_E_INVAL;
+  if (p->min_len < 0 || p->min_len > p->read_len) return DAKC_E_INVAL;
+  if (p->genome_len && p->genome_len < (uint64_t)p->read_len) return DAKC_E_INVAL;
+  if (p->sub_rate < 0 || p->sub_rate > 1 || p->n_rate < 0 || p->n_rate > 1) return DAKC_E_INVAL;
+  if (p->n_rate > 0 && (p->n_run_max < 1 || p->n_run_max > 4096)) return DAKC_E_INVAL;
+  GenCore &c = *g;
+  c.key_iid = gen_stream_key(p->seed, 0);
+  c.key_genome = gen_stream_key(p->seed, 1);
+  c.key_start = gen_stream_key(p->seed, 2);
+  c.key_strand = gen_stream_key(p->seed, 3);
+  c.key_err = gen_stream_key(p->seed, 4);
+  c.key_nrun = gen_stream_key(p->seed, 5);
+  c.key_len = gen_stream_key(p->seed, 6);
+  c.num_reads = p->num_reads;
+  c.first_read = p->first_read;
+  c.genome_len = p->genome_len;
+  auto thr = [](double r) -> uint32_t {
+    double t = r * 4294967296.0;
+    if (t >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)t;
+  };
+  c.sub_thr = thr(p->sub_rate);
+  c.n_thr = thr(p->n_rate);
+  c.L = (uint32_t)p->read_len;
+  c.min_len = (uint32_t)(p->min_len ? p->min_len : p->read_len);
+  c.n_run_max = (uint32_t)std::max(p->n_run_max, 1);
+  c.both_strands = p->both_strands ? 1u : 0u;
+  return DAKC_OK;
+}
+
+}  // namespace dk
+
+extern "C" uint64_t dakc_gen_nbytes(const dakc_gen_params *p) {
+  if (!p || p->read_len < 1) return 0;
+  return p->num_reads * ((uint64_t)p->read_len + 1);
+}
+
+extern "C" int dakc_generate_reads(const dakc_gen_params *p, char *out, uint64_t cap, int threads) {
+  GenCore g;
+  int s = make_gen_core(p, &g);
+  if (s) return s;
+  const uint64_t nb = dakc_gen_nbytes(p);
+  if (!out || cap < nb) return DAKC_E_INVAL;
+  if (threads < 1) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  threads = std::min<int>(threads, 64);
+  const uint64_t R = p->num_reads, stride = (uint64_t)p->read_len + 1;
+  auto work = [&](uint64_t r0, uint64_t r1) {
+    for (uint64_t r = r0; r < r1; ++r) {
+      const uint64_t gr = g.first_read + r;
+      uint8_t *rec = (uint8_t *)out + r * stride;
+      const GenRead ri = gen_read_info(g, gr);
+      for (uint32_t j = 0; j < g.L; ++j) {
+        if (j >= ri.len) rec[j] = 'M';
+        else if (gen_is_n(g, gr, j)) rec[j] = 'N';
+        else rec[j] = (uint8_t)(0x54474341u >> (8 * gen_base(g, gr, j, ri)));
+      }
+      rec[g.L] = '\n';
+    }
+  };
+  if (threads == 1 || R < 1024) {
+    work(0, R);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; ++t) th.emplace_back(work, R * t / threads, R * (t + 1) / threads);
+    for (auto &t : th) t.join();
+  }
+  return DAKC_OK;
+}
+
+__global__ void k_gen_reads(GenCore g, uint8_t *out, uint64_t nbytes) {
+  const uint64_t nw = (nbytes + 3) / 4;
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nw;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t o = 4 * q;
+    if (o + 4 <= nbytes) {
+      uint32_t w = (uint32_t)gen_byte(g, o) | ((uint32_t)gen_byte(g, o + 1) << 8) |
+                   ((uint32_t)gen_byte(g, o + 2) << 16) | ((uint32_t)gen_byte(g, o + 3) << 24);
+      reinterpret_cast<uint32_t *>(out)[q] = w;
+    } else {
+      for (uint64_t b = o; b < nbytes; ++b) out[b] = gen_byte(g, b);
+    }
+  }
+}
+
+extern "C" int dakc_generate_reads_device(dakc_ctx *ctx, const dakc_gen_params *p, void *dev_out,
+                                          uint64_t cap) {
+  if (!ctx) return DAKC_E_INVAL;
+  GenCore g;
+  int s = make_gen_core(p, &g);
+  if (s) return s;
+  const uint64_t nb = dakc_gen_nbytes(p);
+  if (!dev_out || cap < nb || ((uintptr_t)dev_out & 3)) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  if (nb) {
+    const uint64_t nw = (nb + 3) / 4;
+    const uint64_t grid = std::min<uint64_t>((nw + 255) / 256, (uint64_t)ctx->num_cus * 64);
+    k_gen_reads<<<(uint32_t)grid, 256, 0, ctx->stream>>>(g, (uint8_t *)dev_out, nb);
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return DAKC_E_DEVICE;
+  return DAKC_OK;
+}

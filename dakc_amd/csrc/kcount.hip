@@ -1,0 +1,566 @@
+// kcount.hip -- gfx950 kernels of the k-mer counting hot path.
+//
+// Pipeline for one job (replaces kcounter.cpp:247-414, HITTER=0):
+//   k_hist_reads  read records once: every radix digit histogram of every
+//                 k-mer (+ owner digit), record-format checks
+//   k_scan_digits exclusive digit starts, n
+//   k_sweep<from_reads=true>   LSD pass 0: decode + 2-bit pack + sliding
+//                 window straight from the records into digit-0 order
+//   k_sweep<from_reads=false>  remaining LSD passes (onesweep: stable wave
+//                 ballot ranking, decoupled look-back, LDS-staged scatter)
+//   k_rle         run-length reduce of the sorted keys -> (kmer, count)
+//   k_bin_offsets owner-bin boundaries when all P owner bins are counted
+#include "dk_device.hpp"
+#include "dk_kernels.hpp"
+
+namespace dk {
+
+// ------------------------------------------------------------- utilities --
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Exclusive scan of 256 u32 values held by threads [0, 256); all threads call.
+// Returns the exclusive prefix for threads < 256; total in *s_total.
+template <int NT>
+DK_DEV uint32_t scan256(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
+  static_assert(NT >= 256, "needs 256 threads");
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  uint32_t x = tid < 256 ? v : 0;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (tid < 256 && lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  if (tid < 256)
+    for (uint32_t w = 0; w < wave; ++w) pre += s_wave[w];
+  if (tid == 255) *s_total = pre + x;
+  __syncthreads();
+  return pre + x - (tid < 256 ? v : 0);
+}
+
+// Exclusive scan over all NT threads of one u32 each.
+template <int NT>
+DK_DEV uint32_t scan_block(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t w = 0; w < wave; ++w) pre += s_wave[w];
+  if (tid == NT - 1) *s_total = pre + x;
+  __syncthreads();
+  return pre + x - v;
+}
+
+// ------------------------------------------------------------ extraction --
+
+constexpr int MAX_RT = 128;          // records per tile
+constexpr int MAX_CODE_WORDS = 896;  // >= MAX_RT * wpr for every allowed L
+
+struct ExtractSmem {
+  uint32_t raw[RAW_CAP / 4 + 4];
+  uint64_t code[MAX_CODE_WORDS];  // MSB-first 2-bit codes, 32 bases per word
+  uint32_t inv[MAX_CODE_WORDS];   // LSB-first: bit q set = base 32w+q cannot be in a k-mer
+  uint32_t mpos[MAX_RT];          // first 'M' (end of read) per record, L if none
+};
+
+// Decode one tile of records and emit the k-mers of ITEMS slots per thread.
+// Slot s of the tile is (record s / S, start position s % S); thread t holds
+// slots wave*ITEMS*64 + j*64 + lane.  Restates kcounter_funcs.cpp:40-111: a
+// window is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it
+// starts before the record's first 'M' (0xF0 ends the read).
+template <typename KeyT, int NT, int ITEMS, bool CHECK>
+DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, ExtractSmem &sm, KeyT (&keys)[ITEMS],
+                         uint32_t &vmask, uint32_t *err) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t r0 = (uint64_t)tile * rd.RT;
+  const uint64_t left = rd.nrec - r0;
+  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
+  const uint64_t b0 = r0 * rd.stride;
+  uint64_t b1 = b0 + (uint64_t)nr * rd.stride;
+  if (b1 > rd.nbytes) b1 = rd.nbytes;
+  const uint64_t a0 = b0 & ~3ULL;
+  const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
+  const uint32_t *src32 = reinterpret_cast<const uint32_t *>(rd.reads + a0);
+  for (uint32_t i = tid; i < nwords; i += NT) {
+    const uint64_t addr = a0 + 4ULL * i;
+    uint32_t w;
+    if (addr + 4 <= rd.nbytes) {
+      w = src32[i];
+    } else {
+      w = 0;
+      for (uint32_t b = 0; b < 4; ++b)
+        if (addr + b < rd.nbytes) w |= (uint32_t)rd.reads[addr + b] << (8 * b);
+    }
+    sm.raw[i] = w;
+  }
+  for (uint32_t r = tid; r < nr; r += NT) sm.mpos[r] = rd.L;
+  __syncthreads();
+
+  const uint8_t *raw = reinterpret_cast<const uint8_t *>(sm.raw) + (b0 - a0);
+  const uint32_t wpl = (rd.L + 31) >> 5;
+  for (uint32_t it = tid; it < nr * rd.wpr; it += NT) {
+    const uint32_t r = it / rd.wpr, w = it - r * rd.wpr;
+    uint64_t code = 0;
+    uint32_t inv = 0xFFFFFFFFu;
+    if (w < wpl) {
+      const uint8_t *rec = raw + (uint64_t)r * rd.stride;
+      const uint32_t j0 = w * 32;
+      uint32_t mloc = 0xFFFFFFFFu;
+      inv = 0;
+#pragma unroll 8
+      for (uint32_t q = 0; q < 32; ++q) {
+        const uint32_t j = j0 + q;
+        const uint32_t b = j < rd.L ? rec[j] : 0u;
+        const uint32_t u = b & 0xDFu;  // upper-case letters
+        const bool ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T');
+        const uint32_t t = (b >> 1) & 3u;             // A0 C1 T2 G3 (ASCII bits 1-2)
+        const uint32_t c = t ^ ((t >> 1) ^ 1u);       // -> C0 A1 T2 G3
+        code = (code << 2) | (ok ? c : 0u);
+        inv |= (ok ? 0u : 1u) << q;
+        if (u == 'M' && mloc == 0xFFFFFFFFu) mloc = j;
+      }
+      if (mloc != 0xFFFFFFFFu) atomicMin(&sm.mpos[r], mloc);
+      if (CHECK) {
+        if (w == 0 && rec[0] == 0) atomicOr(err, ERR_NUL_RECORD);
+        if (w == wpl - 1 && b0 + (uint64_t)(r + 1) * rd.stride <= rd.nbytes && rec[rd.L] != '\n')
+          atomicOr(err, ERR_SEPARATOR);
+      }
+    }
+    sm.code[it] = code;
+    sm.inv[it] = inv;
+  }
+  __syncthreads();
+
+  const uint32_t lane = __lane_id(), wave = tid >> 6;
+  vmask = 0;
+  if (rd.S == 0) return;  // reads shorter than k: no k-mer slots
+  const int k = (int)rd.k;
+  const uint64_t kmask = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t slot = wave * ITEMS * 64 + j * 64 + lane;
+    uint32_t r = (uint32_t)((float)slot * rd.invS);
+    if (r * rd.S > slot) --r;
+    else if ((r + 1) * rd.S <= slot) ++r;
+    const uint32_t p = slot - r * rd.S;
+    bool v = r < nr;
+    KeyT key{};
+    if (v) {
+      const uint32_t base = r * rd.wpr + (p >> 5), sh = p & 31;
+      const uint64_t m01 = (uint64_t)sm.inv[base] | ((uint64_t)sm.inv[base + 1] << 32);
+      const uint64_t m2 = sm.inv[base + 2];
+      const uint64_t win = (m01 >> sh) | (sh ? (m2 << (64 - sh)) : 0ULL);
+      v = ((win & kmask) == 0) && (p + rd.k <= sm.mpos[r]);
+      // kcounter_funcs.cpp:97: the run left open at the end of a record is
+      // emitted only if it starts before READLEN-1, so a lone base in the last
+      // position never forms a 1-mer
+      if (k == 1 && p == rd.L - 1 && (rd.L < 2 || ((sm.inv[r * rd.wpr + ((p - 1) >> 5)] >> ((p - 1) & 31)) & 1)))
+        v = false;
+      if (v) {
+        key = window_key<KeyT>(sm.code[base], sm.code[base + 1], sm.code[base + 2], 2 * sh, k);
+        if (rd.canonical) key = canon(key, k);
+        if (rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes) != (uint32_t)rd.owner_bin)
+          v = false;
+      }
+    }
+    keys[j] = key;
+    vmask |= (v ? 1u : 0u) << j;
+  }
+}
+
+// ------------------------------------------------------------- histogram --
+
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_hist_reads(ReadsDesc rd, int ndig, int owner_digit,
+                                                   uint32_t ntiles, uint64_t *hist, uint32_t *err) {
+  __shared__ ExtractSmem ex;
+  __shared__ uint32_t s_h[MAX_PASSES * 256];
+  const int nh = (ndig + owner_digit) * 256;
+  for (int i = threadIdx.x; i < nh; i += NT) s_h[i] = 0;
+  // (the first barrier inside extract_tile orders the clear before use)
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    KeyT keys[ITEMS];
+    uint32_t vmask;
+    extract_tile<KeyT, NT, ITEMS, true>(rd, tile, ex, keys, vmask, err);
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (!((vmask >> j) & 1)) continue;
+      for (int p = 0; p < ndig; ++p) atomicAdd(&s_h[p * 256 + key_digit(keys[j], 8 * p)], 1u);
+      if (owner_digit) atomicAdd(&s_h[ndig * 256 + owner_of(keys[j], rd.num_pes)], 1u);
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nh; i += NT)
+    if (s_h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(&hist[i]), (unsigned long long)s_h[i]);
+}
+
+__global__ __launch_bounds__(256) void k_scan_digits(int npasses, const uint64_t *hist,
+                                                     uint64_t *digit_start, uint64_t *n_dev) {
+  __shared__ uint64_t s[256];
+  const int d = threadIdx.x;
+  for (int p = 0; p < npasses; ++p) {
+    s[d] = hist[p * 256 + d];
+    __syncthreads();
+    if (d == 0) {
+      uint64_t run = 0;
+      for (int i = 0; i < 256; ++i) {
+        uint64_t c = s[i];
+        s[i] = run;
+        run += c;
+      }
+      if (p == 0) n_dev[0] = run;
+    }
+    __syncthreads();
+    digit_start[p * 256 + d] = s[d];
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------- LSD pass --
+
+struct SweepArgs {
+  ReadsDesc rd;
+  const void *in;
+  void *out;
+  const uint64_t *n_dev;
+  const uint64_t *digit_start;
+  uint64_t *flags;
+  uint32_t *ticket;
+  uint32_t shift;
+  uint32_t owner_pass;
+  uint32_t num_pes;
+  uint32_t epoch;
+};
+
+template <typename KeyT, int NT, int ITEMS, bool FROM_READS>
+__global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
+  constexpr int WAVES = NT / 64;
+  constexpr int TILE = NT * ITEMS;
+  struct SortSmem {
+    KeyT keys[TILE];
+    uint8_t dig[TILE];
+  };
+  union Smem {
+    ExtractSmem ex;
+    SortSmem so;
+  };
+  static_assert(!FROM_READS || sizeof(ExtractSmem) <= sizeof(SortSmem), "LDS union");
+  __shared__ Smem sm;
+  __shared__ uint16_t s_wh[WAVES * 256];
+  __shared__ uint32_t s_cnt[256], s_bbase[256], s_wave[WAVES];
+  __shared__ uint64_t s_gbase[256];
+  __shared__ uint32_t s_tile, s_total;
+
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+  for (uint32_t i = tid; i < WAVES * 256; i += NT) s_wh[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  uint64_t ntiles, n = 0;
+  if (FROM_READS) {
+    ntiles = (a.rd.nrec + a.rd.RT - 1) / a.rd.RT;
+  } else {
+    n = a.n_dev[0];
+    ntiles = (n + TILE - 1) / TILE;
+  }
+  if (tile >= ntiles) return;
+
+  KeyT keys[ITEMS];
+  uint32_t vmask = 0;
+  if (FROM_READS) {
+    extract_tile<KeyT, NT, ITEMS, false>(a.rd, tile, sm.ex, keys, vmask, nullptr);
+  } else {
+    const KeyT *in = reinterpret_cast<const KeyT *>(a.in);
+    const uint64_t t0 = (uint64_t)tile * TILE + wave * ITEMS * 64 + lane;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint64_t idx = t0 + j * 64;
+      if (idx < n) {
+        keys[j] = in[idx];
+        vmask |= 1u << j;
+      } else {
+        keys[j] = KeyT{};
+      }
+    }
+  }
+
+  // stable rank within the tile: order (wave, item, lane); wave-level digit
+  // match by 8 ballots, running per-wave digit counts in LDS
+  uint16_t rank[ITEMS];
+  uint8_t digs[ITEMS];
+  uint16_t *wh = s_wh + wave * 256;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const bool v = (vmask >> j) & 1;
+    uint32_t d = 0;
+    if (v) d = a.owner_pass ? owner_of(keys[j], a.num_pes) : key_digit(keys[j], a.shift);
+    uint64_t peers = __ballot(v);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below = __popcll(peers & lanemask_lt());
+    uint32_t c = 0;
+    if (v) c = wh[d];
+    if (v && below == 0) wh[d] = (uint16_t)(c + __popcll(peers));
+    rank[j] = (uint16_t)(c + below);
+    digs[j] = (uint8_t)d;
+  }
+  __syncthreads();
+
+  if (tid < 256) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wh[w * 256 + tid];
+      s_wh[w * 256 + tid] = (uint16_t)sum;
+      sum += c;
+    }
+    s_cnt[tid] = sum;
+  }
+  __syncthreads();
+  const uint32_t cnt = tid < 256 ? s_cnt[tid] : 0;
+  const uint32_t bb = scan256<NT>(cnt, s_wave, &s_total);
+  if (tid < 256) {
+    s_bbase[tid] = bb;
+    uint64_t *fl = a.flags + (uint64_t)tile * 256;
+    uint64_t excl = 0;
+    if (tile == 0) {
+      lb_store(fl + tid, lb_pack(a.epoch, LB_INC, cnt));
+    } else {
+      lb_store(fl + tid, lb_pack(a.epoch, LB_AGG, cnt));
+      excl = lb_lookback(a.flags, tile, 256, tid, a.epoch);
+      lb_store(fl + tid, lb_pack(a.epoch, LB_INC, excl + cnt));
+    }
+    s_gbase[tid] = a.digit_start[tid] + excl;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    if ((vmask >> j) & 1) {
+      const uint32_t d = digs[j];
+      const uint32_t pos = s_bbase[d] + s_wh[wave * 256 + d] + rank[j];
+      sm.so.keys[pos] = keys[j];
+      sm.so.dig[pos] = (uint8_t)d;
+    }
+  }
+  __syncthreads();
+  KeyT *out = reinterpret_cast<KeyT *>(a.out);
+  const uint32_t total = s_total;
+  for (uint32_t i = tid; i < total; i += NT) {
+    const uint32_t d = sm.so.dig[i];
+    out[s_gbase[d] + (i - s_bbase[d])] = sm.so.keys[i];
+  }
+}
+
+// ------------------------------------------------------ run-length reduce --
+
+// kcounter.cpp:389-411: collapse equal neighbours of the sorted keys.
+// Each tile emits one (kmer, count) per run that starts inside it; a run
+// that continues past the tile end is measured by a galloping search.
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint32_t *out_counts,
+                                            uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
+                                            uint32_t epoch) {
+  constexpr int TILE = NT * ITEMS;
+  __shared__ KeyT s_k[TILE];
+  __shared__ uint16_t s_hp[TILE + 1];
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_tile;
+  __shared__ uint64_t s_excl;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t n = n_dev[0];
+  const uint64_t ntiles = (n + TILE - 1) / TILE;
+  if (tile >= ntiles) return;
+  const uint64_t t0 = (uint64_t)tile * TILE;
+  const uint32_t cnt = (uint32_t)((n - t0) < (uint64_t)TILE ? (n - t0) : TILE);
+  for (uint32_t i = tid; i < cnt; i += NT) s_k[i] = in[t0 + i];
+  __syncthreads();
+
+  uint32_t heads = 0;
+  bool hf[ITEMS];
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t i = tid * ITEMS + q;
+    bool h = false;
+    if (i < cnt) {
+      if (i > 0) h = !key_eq(s_k[i], s_k[i - 1]);
+      else h = (t0 == 0) || !key_eq(s_k[0], in[t0 - 1]);
+    }
+    hf[q] = h;
+    heads += h;
+  }
+  const uint32_t off = scan_block<NT>(heads, s_wave, &s_total);
+  const uint32_t H = s_total;
+  {
+    uint32_t o = off;
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q)
+      if (hf[q]) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
+  }
+  if (tid == 0) {
+    uint64_t excl = 0;
+    if (tile == 0) {
+      lb_store(flags, lb_pack(epoch, LB_INC, H));
+    } else {
+      lb_store(flags + tile, lb_pack(epoch, LB_AGG, H));
+      excl = lb_lookback(flags, tile, 1, 0, epoch);
+      lb_store(flags + tile, lb_pack(epoch, LB_INC, excl + H));
+    }
+    s_excl = excl;
+    if (tile == ntiles - 1) n_dev[1] = excl + H;
+  }
+  __syncthreads();
+  const uint64_t excl = s_excl;
+  for (uint32_t h = tid; h < H; h += NT) {
+    const uint32_t i = s_hp[h];
+    const KeyT key = s_k[i];
+    uint64_t end;
+    if (h + 1 < H) {
+      end = t0 + s_hp[h + 1];
+    } else {
+      uint64_t lo = t0 + cnt;  // first index past the tile
+      if (lo >= n || !key_eq(in[lo], key)) {
+        end = lo;
+      } else {  // run continues: gallop then bisect for its end
+        uint64_t step = 1, hi;
+        for (;;) {
+          const uint64_t pr = lo + step;
+          if (pr >= n) { hi = n; break; }
+          if (!key_eq(in[pr], key)) { hi = pr; break; }
+          lo = pr;
+          step <<= 1;
+        }
+        while (hi - lo > 1) {
+          const uint64_t mid = lo + ((hi - lo) >> 1);
+          if (key_eq(in[mid], key)) lo = mid;
+          else hi = mid;
+        }
+        end = hi;
+      }
+    }
+    const uint64_t c = end - (t0 + i);
+    out_keys[excl + h] = key;
+    out_counts[excl + h] = c > 0xFFFFFFFFULL ? 0xFFFFFFFFu : (uint32_t)c;
+  }
+}
+
+// owner-bin boundaries of a table sorted by (owner, kmer): lower bound search
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uint64_t *n_dev,
+                                                     uint32_t num_pes, uint64_t *offsets) {
+  const uint64_t d = n_dev[1];
+  for (uint32_t o = threadIdx.x; o <= num_pes; o += blockDim.x) {
+    uint64_t lo = 0, hi = d;
+    if (o == num_pes) lo = d;
+    while (lo < hi) {
+      const uint64_t mid = lo + ((hi - lo) >> 1);
+      if (owner_of(keys[mid], num_pes) < o) lo = mid + 1;
+      else hi = mid;
+    }
+    offsets[o] = lo;
+  }
+}
+
+// ------------------------------------------------------------- launchers --
+
+void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+  if (!n) return;
+  uint64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  k_fill_u32<<<(uint32_t)g, 256, 0, st>>>(p, v, n);
+}
+void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
+  if (!n) return;
+  uint64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  k_fill_u64<<<(uint32_t)g, 256, 0, st>>>(p, v, n);
+}
+
+void launch_hist_reads(int kw, const ReadsDesc &rd, int ndig, int owner_digit, const Workspace &ws,
+                       uint32_t ntiles, int grid, hipStream_t st) {
+  if (kw == 1)
+    k_hist_reads<uint64_t, SWEEP_NT, SWEEP_ITEMS_1>
+        <<<grid, SWEEP_NT, 0, st>>>(rd, ndig, owner_digit, ntiles, ws.hist, ws.err);
+  else
+    k_hist_reads<K2, SWEEP_NT, SWEEP_ITEMS_2>
+        <<<grid, SWEEP_NT, 0, st>>>(rd, ndig, owner_digit, ntiles, ws.hist, ws.err);
+}
+
+void launch_scan_digits(int npasses, const Workspace &ws, hipStream_t st) {
+  k_scan_digits<<<1, 256, 0, st>>>(npasses, ws.hist, ws.digit_start, ws.n_dev);
+}
+
+void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, void *out,
+                  uint32_t shift, bool owner_pass, uint32_t num_pes, const uint64_t *digit_start,
+                  const Workspace &ws, uint32_t ticket_slot, uint32_t epoch, uint32_t grid,
+                  hipStream_t st) {
+  SweepArgs a;
+  a.rd = rd;
+  a.in = in;
+  a.out = out;
+  a.n_dev = ws.n_dev;
+  a.digit_start = digit_start;
+  a.flags = ws.flags;
+  a.ticket = ws.tickets + ticket_slot;
+  a.shift = shift;
+  a.owner_pass = owner_pass ? 1 : 0;
+  a.num_pes = num_pes;
+  a.epoch = epoch;
+  if (!grid) return;
+  if (kw == 1) {
+    if (from_reads) k_sweep<uint64_t, SWEEP_NT, SWEEP_ITEMS_1, true><<<grid, SWEEP_NT, 0, st>>>(a);
+    else k_sweep<uint64_t, SWEEP_NT, SWEEP_ITEMS_1, false><<<grid, SWEEP_NT, 0, st>>>(a);
+  } else {
+    if (from_reads) k_sweep<K2, SWEEP_NT, SWEEP_ITEMS_2, true><<<grid, SWEEP_NT, 0, st>>>(a);
+    else k_sweep<K2, SWEEP_NT, SWEEP_ITEMS_2, false><<<grid, SWEEP_NT, 0, st>>>(a);
+  }
+}
+
+void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
+                uint32_t grid, hipStream_t st) {
+  if (!grid) return;
+  uint32_t *ticket = ws.tickets + MAX_PASSES;
+  if (kw == 1)
+    k_rle<uint64_t, SWEEP_NT, SWEEP_ITEMS_1><<<grid, SWEEP_NT, 0, st>>>(
+        (const uint64_t *)sorted, (uint64_t *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket,
+        epoch);
+  else
+    k_rle<K2, SWEEP_NT, SWEEP_ITEMS_2><<<grid, SWEEP_NT, 0, st>>>(
+        (const K2 *)sorted, (K2 *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket, epoch);
+}
+
+void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,
+                        hipStream_t st) {
+  if (kw == 1)
+    k_bin_offsets<uint64_t><<<1, 256, 0, st>>>((const uint64_t *)table_keys, ws.n_dev, num_pes,
+                                               ws.bin_offsets);
+  else
+    k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
+}
+
+}  // namespace dk

@@ -1,0 +1,223 @@
+"""Python face of the counting path, mirroring the reference's driver surface.
+
+The reference drives one PE as (dakc/src/main/main.cpp:40-49)
+
+    fqreader fq(file, READLEN, rank, size);  char* chunk = fq.read_file();
+    kmercounter km(chunk, vectordbg);         // runs perform_kcount()
+
+with KMERLEN / READLEN fixed at compile time (dakc/Makefile:11).  Here the same
+steps are `FqReader(path, read_len).read_file()` and
+`KmerCounter(k=..., read_len=...).count(chunk)`, with k and read_len chosen at
+run time; the result is the sorted (kmer, count) table the reference keeps in
+`lightdbg` (kcounter.hpp:76-79, kcounter.cpp:385-411).  Every call goes through
+the C ABI of libdakc_amd.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import Job, Opts, Stats, Table, check
+
+
+@dataclass
+class KmerTable:
+    """Sorted (kmer, count) table.  keys: (n, key_words) uint64 (lo, hi for
+    k > 32); counts: (n,) uint32; bin_offsets: owner-bin boundaries when all P
+    bins were counted."""
+    keys: np.ndarray
+    counts: np.ndarray
+    k: int
+    total: int
+    bin_offsets: np.ndarray | None = None
+
+    @property
+    def key_words(self) -> int:
+        return self.keys.shape[1]
+
+    def __len__(self) -> int:
+        return int(self.counts.shape[0])
+
+    def serialize(self) -> bytes:
+        """Little-endian records: u64 kmer (or u64 lo, u64 hi) then u32 count."""
+        kw = self.key_words
+        rec = np.zeros(len(self), dtype=[("k", "<u8", (kw,)), ("c", "<u4")])
+        rec["k"] = self.keys
+        rec["c"] = self.counts
+        return rec.tobytes()
+
+
+class FqReader:
+    """fqreader (dakc/src/fqreader/fqreader.cpp:14-90) for the single-PE case:
+    the whole file of read_len-byte records, each followed by '\\n'."""
+
+    def __init__(self, filename: str, read_len: int):
+        self.filename = filename
+        self.read_len = read_len
+
+    def read_file(self) -> np.ndarray:
+        try:
+            return np.fromfile(self.filename, dtype=np.uint8)
+        except OSError as e:  # the reference prints and exits(2) (fqreader.cpp:31-36)
+            raise FileNotFoundError(f"could not open the input file {self.filename}") from e
+
+
+def _as_bytes(reads) -> np.ndarray:
+    if isinstance(reads, np.ndarray):
+        return np.ascontiguousarray(reads).view(np.uint8).reshape(-1)
+    if isinstance(reads, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(reads), dtype=np.uint8)
+    raise TypeError("reads must be bytes or a uint8 numpy array")
+
+
+class KmerCounter:
+    """One GPU context; count() replaces kmercounter(read_chunk, vectordbg)."""
+
+    def __init__(self, k: int = 31, read_len: int = 150, *, canonical: bool = False,
+                 num_pes: int = 1, owner_bin: int = -1, device: int = 0,
+                 hbm_budget: int = 0, max_inflight: int = 0):
+        self.k, self.read_len, self.canonical = k, read_len, canonical
+        self.num_pes, self.owner_bin = num_pes, owner_bin
+        self._libh = _lib.load()
+        self._ctx = C.c_void_p()
+        opts = Opts(hbm_budget=hbm_budget, max_inflight=max_inflight)
+        check(self._libh.dakc_ctx_create(device, C.byref(opts), C.byref(self._ctx)),
+              "dakc_ctx_create")
+
+    def close(self) -> None:
+        if self._ctx:
+            self._libh.dakc_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._ctx
+
+    def job(self, reads_ptr: int, nbytes: int, *, on_device: bool = False, **over) -> Job:
+        p = dict(k=self.k, read_len=self.read_len, canonical=self.canonical,
+                 num_pes=self.num_pes, owner_bin=self.owner_bin)
+        p.update(over)
+        return Job(reads=reads_ptr, nbytes=nbytes, format=_lib.FMT_FIXED, read_len=p["read_len"],
+                   k=p["k"], canonical=1 if p["canonical"] else 0, num_pes=p["num_pes"],
+                   owner_bin=p["owner_bin"], reads_on_device=1 if on_device else 0)
+
+    def count(self, reads, **over) -> KmerTable:
+        buf = _as_bytes(reads)
+        job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+        t = Table()
+        check(self._libh.dakc_count(self._ctx, C.byref(job), C.byref(t)), "dakc_count")
+        try:
+            keys, counts, offs = _lib.table_to_numpy(t)
+            return KmerTable(keys, counts, int(t.k), int(t.total), offs)
+        finally:
+            self._libh.dakc_table_free(C.byref(t))
+
+    def count_device(self, dev_ptr: int, nbytes: int, **over) -> Table:
+        """Count reads already resident in device memory; the table stays on
+        the device (valid until the next call).  Returns the raw Table."""
+        job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        t = Table()
+        check(self._libh.dakc_count_device(self._ctx, C.byref(job), C.byref(t)), "dakc_count_device")
+        return t
+
+    def last_stats(self) -> Stats:
+        s = Stats()
+        check(self._libh.dakc_last_stats(self._ctx, C.byref(s)), "dakc_last_stats")
+        return s
+
+    def count_batch(self, read_sets, **over) -> list[KmerTable]:
+        bufs = [_as_bytes(r) for r in read_sets]
+        jobs = (Job * len(bufs))(*[self.job(b.ctypes.data if b.size else 0, int(b.size), **over)
+                                   for b in bufs])
+        outs = (Table * len(bufs))()
+        st = self._libh.dakc_count_batch(self._ctx, jobs, len(bufs), outs)
+        try:
+            check(st, "dakc_count_batch")
+            res = []
+            for t in outs:
+                keys, counts, offs = _lib.table_to_numpy(t)
+                res.append(KmerTable(keys, counts, int(t.k), int(t.total), offs))
+            return res
+        finally:
+            for t in outs:
+                self._libh.dakc_table_free(C.byref(t))
+
+    # served mode
+    def submit(self, reads, **over) -> tuple[int, np.ndarray]:
+        """Queue a job; returns (ticket, buffer) -- keep the buffer alive until wait()."""
+        buf = _as_bytes(reads)
+        job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+        tk = C.c_uint64()
+        check(self._libh.dakc_submit(self._ctx, C.byref(job), C.byref(tk)), "dakc_submit")
+        return int(tk.value), buf
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int()
+        check(self._libh.dakc_poll(self._ctx, ticket, C.byref(d)), "dakc_poll")
+        return bool(d.value)
+
+    def wait(self, ticket: int) -> KmerTable:
+        t = Table()
+        check(self._libh.dakc_wait(self._ctx, ticket, C.byref(t)), "dakc_wait")
+        try:
+            keys, counts, offs = _lib.table_to_numpy(t)
+            return KmerTable(keys, counts, int(t.k), int(t.total), offs)
+        finally:
+            self._libh.dakc_table_free(C.byref(t))
+
+    # device memory helpers
+    def device_alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        check(self._libh.dakc_device_alloc(self._ctx, nbytes, C.byref(p)), "dakc_device_alloc")
+        return int(p.value)
+
+    def device_free(self, ptr: int) -> None:
+        check(self._libh.dakc_device_free(self._ctx, C.c_void_p(ptr)), "dakc_device_free")
+
+    def generate_device(self, p: _lib.GenParams, dev_ptr: int, cap: int) -> None:
+        check(self._libh.dakc_generate_reads_device(self._ctx, C.byref(p), C.c_void_p(dev_ptr), cap),
+              "dakc_generate_reads_device")
+
+    def d2h(self, dev_ptr: int, nbytes: int, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        if nbytes:
+            check(self._libh.dakc_memcpy_d2h(self._ctx, out.ctypes.data_as(C.c_void_p),
+                                             C.c_void_p(dev_ptr), nbytes), "dakc_memcpy_d2h")
+        return out
+
+    def h2d(self, dev_ptr: int, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes:
+            check(self._libh.dakc_memcpy_h2d(self._ctx, C.c_void_p(dev_ptr),
+                                             arr.ctypes.data_as(C.c_void_p), arr.nbytes),
+                  "dakc_memcpy_h2d")
+
+
+def gpu_available() -> bool:
+    """True when a GPU context can be created (does not initialise torch)."""
+    try:
+        lib = _lib.load()
+    except (FileNotFoundError, OSError):
+        return False
+    ctx = C.c_void_p()
+    st = lib.dakc_ctx_create(int(os.environ.get("DAKC_DEVICE", "0")), None, C.byref(ctx))
+    if st == 0:
+        lib.dakc_ctx_destroy(ctx)
+        return True
+    return False

@@ -1,0 +1,164 @@
+/*
+ * dakc.h -- C ABI of libdakc_amd: DAKC's k-mer counting hot path on one MI355X.
+ *
+ * The reference (Souvadra/dakc) has no plugin API or FFI; its seam is the C++
+ * constructor
+ *     kmercounter(char* read_chunk, std::vector<kmer_t>& vectordbg)
+ *     (dakc/src/kcounter/kcounter.hpp:84-99, called from dakc/src/main/main.cpp:42-49)
+ * which runs the whole count via perform_kcount() (dakc/src/kcounter/kcounter.cpp:247-507)
+ * with k, READLEN and the PE count fixed at compile time (dakc/Makefile:11,
+ * common.hpp:19-52).  These entry points replace that seam with plain
+ * pointers and sizes: read buffer in, k at run time, sorted (kmer, count)
+ * table out.  Errors are status codes; no C++ exception crosses this ABI.
+ *
+ * Semantics (bit-exact with a single-PE reference run):
+ *   - input: records of exactly read_len bytes followed by '\n'
+ *     (fqreader.cpp:43, kcounter_funcs.cpp:102); the last '\n' may be missing;
+ *   - decode: C/c=0 A/a=1 T/t=2 G/g=3; M/m ends the read; any other byte
+ *     splits it (common.hpp:72-92, kcounter_funcs.cpp:86-99);
+ *   - k-mers: forward strand, first base in the highest bits, 2k bits
+ *     (kcounter_funcs.cpp:25-67); canonical = min(kmer, reverse complement)
+ *     on request (no reference counterpart);
+ *   - owner: MurmurHash64A(kmer, 0x9E3779B97F4A7C15) % num_pes
+ *     (kcounter.cpp:29-54); for 2x64-bit keys MurmurHash64A over lo||hi;
+ *   - output: ascending by k-mer (within each owner bin), no duplicates,
+ *     counts >= 1 (kcounter.cpp:385-411).
+ */
+#ifndef DAKC_H
+#define DAKC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DAKC_ABI_VERSION 1
+
+typedef enum dakc_status {
+  DAKC_OK = 0,
+  DAKC_E_INVAL = 1,     /* bad argument (k out of 1..64, read_len, owner bin, NULL) */
+  DAKC_E_FORMAT = 2,    /* buffer is not whole records / separator not '\n' / NUL record */
+  DAKC_E_NOMEM = 3,     /* host or device allocation failed */
+  DAKC_E_TOOBIG = 4,    /* job has 2^32 or more k-mer slots (counts are u32) */
+  DAKC_E_DEVICE = 5,    /* HIP runtime error */
+  DAKC_E_NODEVICE = 6,  /* no usable GPU */
+  DAKC_E_REJECTED = 7,  /* served mode: job can never fit the HBM budget */
+  DAKC_E_NOTFOUND = 8,  /* unknown ticket */
+  DAKC_E_PENDING = 9,   /* served mode: job not finished yet (dakc_poll) */
+  DAKC_E_CANCELLED = 10 /* served mode: job was cancelled */
+} dakc_status;
+
+/* read buffer formats */
+#define DAKC_FMT_FIXED 0 /* read_len bytes + '\n' per record (the reference's .txt) */
+
+typedef struct dakc_job {
+  const char *reads;   /* host pointer, or device pointer when reads_on_device */
+  uint64_t nbytes;
+  int32_t format;      /* DAKC_FMT_FIXED */
+  int32_t read_len;    /* READLEN: record width in bases (1..4096) */
+  int32_t k;           /* KMERLEN: 1..64 (k <= 32: one u64 per key, else two) */
+  int32_t canonical;   /* 0: forward strand (the reference), 1: min(x, revcomp x) */
+  uint32_t num_pes;    /* P; 1 = single PE */
+  int32_t owner_bin;   /* -1: all P owner bins; b >= 0: only k-mers owned by PE b */
+  int32_t reads_on_device; /* reads is a device pointer (4-byte aligned) */
+  int32_t reserved;
+} dakc_job;
+
+/* The (kmer, count) table.  keys holds n * key_words u64 (lo, hi per key for
+ * key_words == 2); counts holds n u32.  With num_pes > 1 and owner_bin == -1
+ * the table is the concatenation of the P owner tables and bin_offsets (P + 1
+ * entries) marks their boundaries; otherwise bin_offsets is NULL.
+ * Memory is owned by the library: release with dakc_table_free. */
+typedef struct dakc_table {
+  uint64_t *keys;
+  uint32_t *counts;
+  uint64_t n;           /* distinct k-mers */
+  uint64_t total;       /* k-mers counted (sum of counts) */
+  uint64_t *bin_offsets;
+  uint32_t num_bins;
+  int32_t key_words;
+  int32_t k;
+  int32_t on_device;    /* 1: keys/counts are device pointers (dakc_count_device) */
+  void *priv;
+} dakc_table;
+
+typedef struct dakc_ctx dakc_ctx;
+
+typedef struct dakc_opts {
+  uint64_t hbm_budget;  /* served mode: device bytes the job pool may hold (0 = 60% of free) */
+  int32_t max_inflight; /* served mode: concurrent jobs on the device (0 = 4) */
+  int32_t reserved;
+} dakc_opts;
+
+/* Stage timings of the last dakc_count/dakc_count_device on a context,
+ * measured with HIP events on the context's stream (milliseconds). */
+typedef struct dakc_stats {
+  double hist_ms, pass0_ms, passes_ms, rle_ms, total_ms;
+  uint32_t npasses;     /* radix passes executed (including pass 0) */
+  uint32_t tiles0;      /* tiles of pass 0 */
+  uint64_t n;           /* k-mers */
+  uint64_t d;           /* distinct */
+  double sweep_ms[18];  /* per radix pass */
+} dakc_stats;
+
+const char *dakc_status_str(int status);
+int dakc_abi_version(void);
+
+int dakc_ctx_create(int device, const dakc_opts *opts, dakc_ctx **out);
+int dakc_ctx_destroy(dakc_ctx *ctx);
+
+/* Plain count: one job, table copied to host memory. */
+int dakc_count(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
+/* Same, table left in device memory owned by the context (valid until the
+ * next call on ctx or dakc_table_free). */
+int dakc_count_device(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
+int dakc_last_stats(dakc_ctx *ctx, dakc_stats *out);
+
+/* Batched: many independent jobs, one table each (outs[i] for jobs[i]). */
+int dakc_count_batch(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_table *outs);
+
+/* Served: continuous batching.  submit copies the job descriptor (the reads
+ * buffer must stay valid until completion); wait blocks for the table. */
+int dakc_submit(dakc_ctx *ctx, const dakc_job *job, uint64_t *ticket);
+int dakc_poll(dakc_ctx *ctx, uint64_t ticket, int *done);
+int dakc_wait(dakc_ctx *ctx, uint64_t ticket, dakc_table *out);
+
+void dakc_table_free(dakc_table *t);
+
+/* Serialize a host table as little-endian records: u64 kmer (k <= 32) or
+ * u64 lo, u64 hi (k <= 64), then u32 count.  Returns bytes written or
+ * needed when buf is NULL. */
+uint64_t dakc_table_serialize(const dakc_table *t, void *buf, uint64_t cap);
+
+/* ------------------------------------------------ synthetic reads ---------
+ * Seeded, counter-based generator of reads in the fixed-width format, so the
+ * same bytes can be made on the host and on the device. */
+typedef struct dakc_gen_params {
+  uint64_t seed;
+  uint64_t num_reads;
+  uint64_t first_read;  /* generate reads [first_read, first_read + num_reads) */
+  uint64_t genome_len;  /* 0: i.i.d. uniform bases; else reads sampled from a genome */
+  double sub_rate;      /* per-base substitution probability */
+  double n_rate;        /* per-position probability that an N run starts */
+  int32_t read_len;     /* record width */
+  int32_t min_len;      /* read lengths uniform in [min_len, read_len]; rest M-padded */
+  int32_t n_run_max;    /* N runs have length uniform in [1, n_run_max] */
+  int32_t both_strands; /* genome reads from either strand */
+} dakc_gen_params;
+
+uint64_t dakc_gen_nbytes(const dakc_gen_params *p);
+int dakc_generate_reads(const dakc_gen_params *p, char *out, uint64_t cap, int threads);
+int dakc_generate_reads_device(dakc_ctx *ctx, const dakc_gen_params *p, void *dev_out, uint64_t cap);
+
+/* device memory helpers for callers without their own allocator */
+int dakc_device_alloc(dakc_ctx *ctx, uint64_t nbytes, void **dptr);
+int dakc_device_free(dakc_ctx *ctx, void *dptr);
+int dakc_memcpy_h2d(dakc_ctx *ctx, void *dst, const void *src, uint64_t nbytes);
+int dakc_memcpy_d2h(dakc_ctx *ctx, void *dst, const void *src, uint64_t nbytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

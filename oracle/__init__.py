@@ -1,0 +1,145 @@
+"""ctypes binding of the CPU oracle (oracle/dakc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product (dakc_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libdakc_oracle.so"
+OWNER_SEED = 0x9E3779B97F4A7C15
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    lib.dko_char2base.restype = C.c_uint8
+    lib.dko_char2base.argtypes = [C.c_ubyte]
+    lib.dko_murmur64a_u64.restype = C.c_uint64
+    lib.dko_murmur64a_u64.argtypes = [C.c_uint64, C.c_uint64]
+    lib.dko_murmur64a_u128.restype = C.c_uint64
+    lib.dko_murmur64a_u128.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    lib.dko_extract.restype = C.c_int64
+    lib.dko_extract.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                C.c_int32, C.c_void_p, C.c_uint64]
+    lib.dko_count.restype = C.c_int
+    lib.dko_count.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                              C.c_int32, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64),
+                              P(C.c_uint64), C.c_void_p]
+    lib.dko_count_mt.restype = C.c_int
+    lib.dko_count_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, P(C.c_uint64),
+                                 P(C.c_uint64), P(C.c_uint64), P(C.c_double)]
+    lib.dko_table_checksum.restype = C.c_uint64
+    lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
+    lib.dko_free.restype = None
+    lib.dko_free.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _buf(reads) -> np.ndarray:
+    if isinstance(reads, np.ndarray):
+        return np.ascontiguousarray(reads).view(np.uint8).reshape(-1)
+    return np.frombuffer(bytes(reads), dtype=np.uint8)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status):
+        self.status = status
+        super().__init__(f"oracle status {status}")
+
+
+def extract(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1) -> np.ndarray:
+    lib = load()
+    b = _buf(reads)
+    ptr = b.ctypes.data if b.size else None
+    n = lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, None, 0)
+    if n < 0:
+        raise OracleError(-n)
+    kw = 1 if k <= 32 else 2
+    out = np.zeros((max(n, 1), kw), np.uint64)
+    lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin,
+                    out.ctypes.data, n)
+    return out[:n]
+
+
+def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1):
+    """Returns (keys (d, kw) u64, counts (d,) u32, total n, bin_offsets or None)."""
+    lib = load()
+    b = _buf(reads)
+    ptr = b.ctypes.data if b.size else None
+    keys = C.POINTER(C.c_uint64)()
+    counts = C.POINTER(C.c_uint32)()
+    d = C.c_uint64()
+    n = C.c_uint64()
+    all_bins = num_pes > 1 and owner_bin < 0
+    offs = np.zeros(num_pes + 1, np.uint64) if all_bins else None
+    st = lib.dko_count(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, C.byref(keys),
+                       C.byref(counts), C.byref(d), C.byref(n),
+                       offs.ctypes.data if offs is not None else None)
+    if st:
+        raise OracleError(st)
+    kw = 1 if k <= 32 else 2
+    dn = int(d.value)
+    try:
+        if dn:
+            kk = np.ctypeslib.as_array(keys, shape=(dn * kw,)).copy().reshape(dn, kw)
+            cc = np.ctypeslib.as_array(counts, shape=(dn,)).copy()
+        else:
+            kk = np.zeros((0, kw), np.uint64)
+            cc = np.zeros(0, np.uint32)
+    finally:
+        lib.dko_free(C.cast(keys, C.c_void_p))
+        lib.dko_free(C.cast(counts, C.c_void_p))
+    return kk, cc, int(n.value), offs
+
+
+def count_mt(reads, read_len, k, threads):
+    """DAKC's P-PE shape on `threads` CPU threads; returns (n, d, checksum, seconds)."""
+    lib = load()
+    b = _buf(reads)
+    n, d, cs, el = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
+    st = lib.dko_count_mt(b.ctypes.data if b.size else None, b.size, read_len, k, threads,
+                          C.byref(n), C.byref(d), C.byref(cs), C.byref(el))
+    if st:
+        raise OracleError(st)
+    return int(n.value), int(d.value), int(cs.value), float(el.value)
+
+
+def checksum(keys: np.ndarray, counts: np.ndarray) -> int:
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    return int(lib.dko_table_checksum(keys.ctypes.data, counts.ctypes.data, counts.size,
+                                      keys.shape[1] if keys.ndim == 2 else 1))
+
+
+def char2base(c: int) -> int:
+    return int(load().dko_char2base(c))
+
+
+def murmur64a_u64(key: int, seed: int = OWNER_SEED) -> int:
+    return int(load().dko_murmur64a_u64(key, seed))
+
+
+def murmur64a_u128(lo: int, hi: int, seed: int = OWNER_SEED) -> int:
+    return int(load().dko_murmur64a_u128(lo, hi, seed))

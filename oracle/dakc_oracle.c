@@ -1,0 +1,446 @@
+/*
+ * dakc_oracle.c -- CPU restatement of DAKC's single-PE k-mer counting path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see dakc_oracle.h): the checker for the HIP path
+ * and the timed CPU baseline.  Written from the behaviour of the reference
+ * (Souvadra/dakc), not copied from it.  Every function cites what it restates.
+ *
+ * Deliberate, documented departures from the reference (its behaviour there is
+ * undefined or reads stale memory):
+ *   - empty input gives an empty table (kcounter.cpp:389-411 would emit one
+ *     stale entry);
+ *   - a buffer whose length is not a whole number of records, a separator byte
+ *     that is not '\n', or a record that starts with NUL is a format error
+ *     (the reference over-reads, mis-parses, or silently stops parsing);
+ *   - a final record without its '\n' is accepted (kcounter_funcs.cpp:102-103
+ *     would read past the buffer).
+ */
+#define _GNU_SOURCE
+#include "dakc_oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+typedef unsigned __int128 u128;
+
+/* common.hpp:72-92 (char2base): C/c->0, A/a->1, T/t->2, G/g->3, M/m->0xF0,
+ * every other byte -> 0xFF.  Written as a switch, not as a copy of the table. */
+uint8_t dko_char2base(unsigned char c) {
+    switch (c) {
+    case 'C': case 'c': return 0;
+    case 'A': case 'a': return 1;
+    case 'T': case 't': return 2;
+    case 'G': case 'g': return 3;
+    case 'M': case 'm': return 0xF0;
+    default: return 0xFF;
+    }
+}
+
+/* kcounter.cpp:29-48 (MurmurHash64A, one 8-byte block, len = 8) */
+uint64_t dko_murmur64a_u64(uint64_t key, uint64_t seed) {
+    const uint64_t m = 0xc6a4a7935bd1e995ULL;
+    const int r = 47;
+    uint64_t h = seed ^ (8 * m);
+    uint64_t k = key * m;
+    k ^= k >> r;
+    k *= m;
+    h ^= k;
+    h *= m;
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+/* MurmurHash64A (Appleby's public algorithm) over the 16 bytes lo||hi. */
+uint64_t dko_murmur64a_u128(uint64_t lo, uint64_t hi, uint64_t seed) {
+    const uint64_t m = 0xc6a4a7935bd1e995ULL;
+    const int r = 47;
+    uint64_t h = seed ^ (16 * m);
+    uint64_t blocks[2] = {lo, hi};
+    for (int b = 0; b < 2; b++) {
+        uint64_t k = blocks[b] * m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ULL;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebULL;
+    x ^= x >> 31;
+    return x;
+}
+
+uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words) {
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < d; i++) {
+        uint64_t lo = keys[i * key_words];
+        uint64_t hi = key_words == 2 ? keys[i * key_words + 1] : 0;
+        s += mix64(mix64(lo + 0x9E3779B97F4A7C15ULL) ^ (hi + 0xD1B54A32D192ED03ULL) ^ ((uint64_t)counts[i] << 17));
+    }
+    return s;
+}
+
+void dko_free(void *p) { free(p); }
+
+/* ---------------------------------------------------------------- format --- */
+
+/* Record layout of fqreader.cpp:43 and kcounter_funcs.cpp:102: read_len bytes
+ * then '\n'.  Returns the record count or -(status). */
+static int64_t count_records(const char *reads, size_t nbytes, int read_len) {
+    if (read_len < 1) return -DKO_E_INVAL;
+    if (nbytes == 0) return 0;
+    if (!reads) return -DKO_E_INVAL;
+    size_t stride = (size_t)read_len + 1;
+    size_t nrec = nbytes / stride;
+    size_t rem = nbytes % stride;
+    if (rem == (size_t)read_len) nrec += 1; /* last record without '\n' */
+    else if (rem != 0) return -DKO_E_FORMAT;
+    for (size_t r = 0; r < nrec; r++) {
+        const char *rec = reads + r * stride;
+        if (rec[0] == '\0') return -DKO_E_FORMAT;
+        if ((r + 1) * stride <= nbytes && rec[read_len] != '\n') return -DKO_E_FORMAT;
+    }
+    return (int64_t)nrec;
+}
+
+static int key_words_for(int k) { return k <= 32 ? 1 : 2; }
+
+static u128 mask_for(int k) {
+    return k >= 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1);
+}
+
+/* Emission context for one run of valid codes. */
+typedef struct {
+    int k, canonical, kw;
+    uint32_t num_pes;
+    int32_t owner_bin;
+    uint64_t *out;
+    uint64_t cap, n;
+} emit_ctx;
+
+static inline uint32_t owner_of(u128 key, int kw, uint32_t num_pes) {
+    uint64_t h = kw == 1 ? dko_murmur64a_u64((uint64_t)key, DKO_OWNER_SEED)
+                         : dko_murmur64a_u128((uint64_t)key, (uint64_t)(key >> 64), DKO_OWNER_SEED);
+    return (uint32_t)(h % num_pes);
+}
+
+static inline void emit_key(emit_ctx *c, u128 key) {
+    if (c->num_pes > 1 && c->owner_bin >= 0 && owner_of(key, c->kw, c->num_pes) != (uint32_t)c->owner_bin)
+        return;
+    if (c->out && c->n < c->cap) {
+        c->out[c->n * c->kw] = (uint64_t)key;
+        if (c->kw == 2) c->out[c->n * c->kw + 1] = (uint64_t)(key >> 64);
+    }
+    c->n++;
+}
+
+/* kcounter_funcs.cpp:40-67 (get_kmers) with set_kmer_fast / update_kmer_fast
+ * (:25-38): the first k-mer of a run is built from k codes, MSB first; each
+ * next k-mer shifts in one code and masks to 2k bits (KMER_MASK common.hpp:61).
+ * Canonical mode (no reference counterpart) keeps min(x, revcomp(x)). */
+static void emit_run(emit_ctx *c, const uint8_t *codes, int len) {
+    int k = c->k;
+    if (len < k) return;
+    const u128 mask = mask_for(k);
+    u128 x = 0, rc = 0;
+    for (int j = 0; j < k; j++) {
+        x = (x << 2) | codes[j];
+        rc = (rc >> 2) | ((u128)(codes[j] ^ 3) << (2 * (k - 1)));
+    }
+    emit_key(c, c->canonical && rc < x ? rc : x);
+    for (int i = k; i < len; i++) {
+        x = ((x << 2) | codes[i]) & mask;
+        rc = (rc >> 2) | ((u128)(codes[i] ^ 3) << (2 * (k - 1)));
+        emit_key(c, c->canonical && rc < x ? rc : x);
+    }
+}
+
+/* kcounter_funcs.cpp:69-111 (read_till_buf_max), one record: decode each byte
+ * (common.hpp:72-92); a 0xFF closes the current run (:88-91); a 0xF0 ('M')
+ * ends the record (:92-94); the final run is emitted at the end (:97-99). */
+static void parse_record(emit_ctx *c, const char *rec, int read_len, uint8_t *codes) {
+    int left = 0, i;
+    for (i = 0; i < read_len; i++) {
+        uint8_t b = dko_char2base((unsigned char)rec[i]);
+        codes[i] = b;
+        if (b == 0xFF) {
+            emit_run(c, codes + left, i - left);
+            left = i + 1;
+        }
+        if (b == 0xF0) break;
+    }
+    if (left < read_len - 1) emit_run(c, codes + left, i - left);
+}
+
+static int64_t extract_range(const char *reads, int read_len, uint64_t r0, uint64_t r1, emit_ctx *c) {
+    uint8_t *codes = (uint8_t *)malloc((size_t)read_len + 1);
+    if (!codes) return -DKO_E_NOMEM;
+    size_t stride = (size_t)read_len + 1;
+    for (uint64_t r = r0; r < r1; r++) parse_record(c, reads + r * stride, read_len, codes);
+    free(codes);
+    return (int64_t)c->n;
+}
+
+int64_t dko_extract(const char *reads, size_t nbytes, int read_len, int k, int canonical,
+                    uint32_t num_pes, int32_t owner_bin, uint64_t *out, uint64_t cap) {
+    if (k < 1 || k > 64 || num_pes < 1) return -DKO_E_INVAL;
+    if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return -DKO_E_INVAL;
+    int64_t nrec = count_records(reads, nbytes, read_len);
+    if (nrec < 0) return nrec;
+    emit_ctx c = {k, canonical, key_words_for(k), num_pes, owner_bin, out, cap, 0};
+    return extract_range(reads, read_len, 0, (uint64_t)nrec, &c);
+}
+
+/* ------------------------------------------------------------------ sort --- */
+
+/* Ascending unsigned LSD radix sort, 8-bit digits, constant digits skipped.
+ * Stands in for ska_sort (kcounter.cpp:388, ska_sort.hpp:1417-1421): the sorted
+ * order of integer keys is unique, so the sort algorithm does not matter. */
+static void sort_u64(uint64_t *a, uint64_t *tmp, uint64_t n, int bits) {
+    uint64_t *src = a, *dst = tmp;
+    for (int sh = 0; sh < bits; sh += 8) {
+        uint64_t cnt[256] = {0};
+        for (uint64_t i = 0; i < n; i++) cnt[(src[i] >> sh) & 255]++;
+        int trivial = 0;
+        for (int d = 0; d < 256; d++) if (cnt[d] == n) trivial = 1;
+        if (trivial) continue;
+        uint64_t s = 0;
+        for (int d = 0; d < 256; d++) { uint64_t t = cnt[d]; cnt[d] = s; s += t; }
+        for (uint64_t i = 0; i < n; i++) dst[cnt[(src[i] >> sh) & 255]++] = src[i];
+        uint64_t *t = src; src = dst; dst = t;
+    }
+    if (src != a) memcpy(a, src, n * sizeof(uint64_t));
+}
+
+/* same for (lo, hi) pairs stored interleaved, ordered by the 128-bit value */
+static void sort_u128(uint64_t *a, uint64_t *tmp, uint64_t n, int bits) {
+    uint64_t *src = a, *dst = tmp;
+    for (int sh = 0; sh < bits; sh += 8) {
+        int w = sh >= 64 ? 1 : 0, s8 = sh & 63;
+        uint64_t cnt[256] = {0};
+        for (uint64_t i = 0; i < n; i++) cnt[(src[2 * i + w] >> s8) & 255]++;
+        int trivial = 0;
+        for (int d = 0; d < 256; d++) if (cnt[d] == n) trivial = 1;
+        if (trivial) continue;
+        uint64_t s = 0;
+        for (int d = 0; d < 256; d++) { uint64_t t = cnt[d]; cnt[d] = s; s += t; }
+        for (uint64_t i = 0; i < n; i++) {
+            uint64_t p = cnt[(src[2 * i + w] >> s8) & 255]++;
+            dst[2 * p] = src[2 * i];
+            dst[2 * p + 1] = src[2 * i + 1];
+        }
+        uint64_t *t = src; src = dst; dst = t;
+    }
+    if (src != a) memcpy(a, src, 2 * n * sizeof(uint64_t));
+}
+
+static void sort_keys(uint64_t *a, uint64_t *tmp, uint64_t n, int k) {
+    if (key_words_for(k) == 1) sort_u64(a, tmp, n, 2 * k);
+    else sort_u128(a, tmp, n, 2 * k);
+}
+
+/* kcounter.cpp:389-411: collapse equal neighbours of a sorted array into
+ * (kmer, count).  Writes in place over `a` (d <= n); returns d.  An empty input
+ * gives d = 0 (the reference would emit a stale entry here). */
+static uint64_t rle(uint64_t *a, uint64_t n, int kw, uint32_t *counts) {
+    if (n == 0) return 0;
+    uint64_t d = 0, run = 1;
+    for (uint64_t i = 1; i <= n; i++) {
+        int same = 0;
+        if (i < n) {
+            same = a[i * kw] == a[(i - 1) * kw];
+            if (kw == 2) same = same && a[i * kw + 1] == a[(i - 1) * kw + 1];
+        }
+        if (same) { run++; continue; }
+        for (int w = 0; w < kw; w++) a[d * kw + w] = a[(i - 1) * kw + w];
+        counts[d++] = run > 0xFFFFFFFFULL ? 0xFFFFFFFFu : (uint32_t)run;
+        run = 1;
+    }
+    return d;
+}
+
+int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonical,
+              uint32_t num_pes, int32_t owner_bin, uint64_t **keys_out, uint32_t **counts_out,
+              uint64_t *d_out, uint64_t *n_out, uint64_t *bin_offsets) {
+    if (!keys_out || !counts_out || !d_out) return DKO_E_INVAL;
+    *keys_out = NULL; *counts_out = NULL; *d_out = 0;
+    int64_t n = dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, NULL, 0);
+    if (n < 0) return (int)-n;
+    if ((uint64_t)n > 0xFFFFFFFFULL) return DKO_E_TOOBIG;
+    int kw = key_words_for(k);
+    uint64_t *keys = (uint64_t *)malloc(((size_t)n + 1) * kw * sizeof(uint64_t));
+    uint64_t *tmp = (uint64_t *)malloc(((size_t)n + 1) * kw * sizeof(uint64_t));
+    uint32_t *counts = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+    if (!keys || !tmp || !counts) { free(keys); free(tmp); free(counts); return DKO_E_NOMEM; }
+    dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, keys, (uint64_t)n);
+    if (n_out) *n_out = (uint64_t)n;
+    uint64_t d = 0;
+    if (num_pes > 1 && owner_bin < 0) {
+        /* all owner bins: partition by owner (kcounter.cpp:162-174 routes each
+         * k-mer to its owner), then each owner sorts and reduces its own bin. */
+        uint64_t *cnt = (uint64_t *)calloc((size_t)num_pes + 1, sizeof(uint64_t));
+        uint32_t *own = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+        if (!cnt || !own) { free(cnt); free(own); free(keys); free(tmp); free(counts); return DKO_E_NOMEM; }
+        for (int64_t i = 0; i < n; i++) {
+            u128 key = keys[i * kw];
+            if (kw == 2) key |= (u128)keys[i * kw + 1] << 64;
+            own[i] = owner_of(key, kw, num_pes);
+            cnt[own[i] + 1]++;
+        }
+        for (uint32_t p = 0; p < num_pes; p++) cnt[p + 1] += cnt[p];
+        uint64_t *pos = (uint64_t *)malloc((size_t)num_pes * sizeof(uint64_t));
+        if (!pos) { free(cnt); free(own); free(keys); free(tmp); free(counts); return DKO_E_NOMEM; }
+        memcpy(pos, cnt, (size_t)num_pes * sizeof(uint64_t));
+        for (int64_t i = 0; i < n; i++) {
+            uint64_t p = pos[own[i]]++;
+            for (int w = 0; w < kw; w++) tmp[p * kw + w] = keys[i * kw + w];
+        }
+        memcpy(keys, tmp, (size_t)n * kw * sizeof(uint64_t));
+        for (uint32_t p = 0; p < num_pes; p++) {
+            uint64_t b0 = cnt[p], b1 = cnt[p + 1];
+            sort_keys(keys + b0 * kw, tmp, b1 - b0, k);
+            uint64_t db = rle(keys + b0 * kw, b1 - b0, kw, counts + d);
+            if (bin_offsets) bin_offsets[p] = d;
+            memmove(keys + d * kw, keys + b0 * kw, db * kw * sizeof(uint64_t));
+            d += db;
+        }
+        if (bin_offsets) bin_offsets[num_pes] = d;
+        free(pos); free(own); free(cnt);
+    } else {
+        sort_keys(keys, tmp, (uint64_t)n, k);
+        d = rle(keys, (uint64_t)n, kw, counts);
+        if (bin_offsets) { bin_offsets[0] = 0; bin_offsets[1] = d; }
+    }
+    free(tmp);
+    *keys_out = keys;
+    *counts_out = counts;
+    *d_out = d;
+    return DKO_OK;
+}
+
+/* -------------------------------------------------- threaded P-PE shape --- */
+
+typedef struct {
+    int tid, T, read_len, k;
+    const char *reads;
+    uint64_t r0, r1;
+    uint64_t *local;      /* this worker's k-mers (phase 1) */
+    uint64_t nlocal;
+    uint64_t *cnt;        /* T x T: cnt[t*T + o] */
+    uint64_t *global, *tmp;
+    uint64_t *bin_start;  /* T + 1 */
+    uint64_t d, checksum;
+    int status;
+    pthread_barrier_t *bar;
+} mt_arg;
+
+static void *mt_worker(void *p) {
+    mt_arg *a = (mt_arg *)p;
+    int T = a->T;
+    /* Phase 1: parse this worker's block of records (fqreader.cpp:43-52) */
+    emit_ctx c = {a->k, 0, 1, 1, -1, a->local, a->nlocal, 0};
+    extract_range(a->reads, a->read_len, a->r0, a->r1, &c);
+    a->nlocal = c.n;
+    uint64_t *mycnt = a->cnt + (size_t)a->tid * T;
+    for (int o = 0; o < T; o++) mycnt[o] = 0;
+    for (uint64_t i = 0; i < c.n; i++) mycnt[dko_murmur64a_u64(a->local[i], DKO_OWNER_SEED) % (uint64_t)T]++;
+    pthread_barrier_wait(a->bar);
+    /* owner bins: bin o holds worker 0's k-mers for o, then worker 1's, ... */
+    uint64_t *pos = (uint64_t *)malloc((size_t)T * sizeof(uint64_t));
+    for (int o = 0; o < T; o++) {
+        uint64_t s = 0;
+        for (int o2 = 0; o2 < o; o2++) for (int t = 0; t < T; t++) s += a->cnt[(size_t)t * T + o2];
+        for (int t = 0; t < a->tid; t++) s += a->cnt[(size_t)t * T + o];
+        pos[o] = s;
+        if (a->tid == 0) a->bin_start[o] = s;
+    }
+    if (a->tid == 0) {
+        uint64_t tot = 0;
+        for (size_t i = 0; i < (size_t)T * T; i++) tot += a->cnt[i];
+        a->bin_start[T] = tot;
+    }
+    for (uint64_t i = 0; i < c.n; i++) {
+        uint64_t x = a->local[i];
+        a->global[pos[dko_murmur64a_u64(x, DKO_OWNER_SEED) % (uint64_t)T]++] = x;
+    }
+    free(pos);
+    pthread_barrier_wait(a->bar);
+    /* Phase 2: each owner sorts and reduces its own bin (kcounter.cpp:385-411) */
+    uint64_t b0 = a->bin_start[a->tid], b1 = a->bin_start[a->tid + 1];
+    sort_u64(a->global + b0, a->tmp + b0, b1 - b0, 2 * a->k);
+    uint32_t *counts = (uint32_t *)(a->tmp + b0); /* reuse scratch */
+    uint64_t d = rle(a->global + b0, b1 - b0, 1, counts);
+    a->d = d;
+    a->checksum = dko_table_checksum(a->global + b0, counts, d, 1);
+    return NULL;
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int threads,
+                 uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s) {
+    if (threads < 1 || threads > 1024 || k < 1 || k > 32) return DKO_E_INVAL;
+    int64_t nrec = count_records(reads, nbytes, read_len);
+    if (nrec < 0) return (int)-nrec;
+    int T = threads;
+    uint64_t per_read = read_len >= k ? (uint64_t)(read_len - k + 1) : 0;
+    uint64_t nmax = (uint64_t)nrec * per_read;
+    uint64_t *local = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
+    uint64_t *global = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
+    uint64_t *tmp = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
+    uint64_t *cnt = (uint64_t *)calloc((size_t)T * T, sizeof(uint64_t));
+    uint64_t *bin_start = (uint64_t *)calloc((size_t)T + 1, sizeof(uint64_t));
+    mt_arg *args = (mt_arg *)calloc((size_t)T, sizeof(mt_arg));
+    pthread_t *th = (pthread_t *)calloc((size_t)T, sizeof(pthread_t));
+    if (!local || !global || !tmp || !cnt || !bin_start || !args || !th) {
+        free(local); free(global); free(tmp); free(cnt); free(bin_start); free(args); free(th);
+        return DKO_E_NOMEM;
+    }
+    /* touch the buffers once so page faults are not timed */
+    memset(local, 0, (size_t)(nmax + 1) * 8);
+    memset(global, 0, (size_t)(nmax + 1) * 8);
+    memset(tmp, 0, (size_t)(nmax + 1) * 8);
+    pthread_barrier_t bar;
+    pthread_barrier_init(&bar, NULL, (unsigned)T);
+    double t0 = now_s();
+    for (int t = 0; t < T; t++) {
+        mt_arg *a = &args[t];
+        a->tid = t; a->T = T; a->read_len = read_len; a->k = k; a->reads = reads;
+        a->r0 = (uint64_t)nrec * t / T;
+        a->r1 = (uint64_t)nrec * (t + 1) / T;
+        a->local = local + a->r0 * per_read;
+        a->nlocal = (a->r1 - a->r0) * per_read;
+        a->cnt = cnt; a->global = global; a->tmp = tmp; a->bin_start = bin_start; a->bar = &bar;
+        pthread_create(&th[t], NULL, mt_worker, a);
+    }
+    uint64_t d = 0, cs = 0;
+    for (int t = 0; t < T; t++) {
+        pthread_join(th[t], NULL);
+        d += args[t].d;
+        cs += args[t].checksum;
+    }
+    double t1 = now_s();
+    pthread_barrier_destroy(&bar);
+    if (n_out) *n_out = bin_start[T];
+    if (d_out) *d_out = d;
+    if (checksum_out) *checksum_out = cs;
+    if (elapsed_s) *elapsed_s = t1 - t0;
+    free(local); free(global); free(tmp); free(cnt); free(bin_start); free(args); free(th);
+    return DKO_OK;
+}

@@ -1,0 +1,90 @@
+/*
+ * dakc_oracle.h -- CPU restatement of DAKC's single-PE k-mer counting path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (dakc_amd/, include/)
+ * links, loads or calls this code.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg use it, and only as the checker / the timed CPU
+ * baseline, never as the thing measured or shipped.
+ *
+ * Each function cites the reference file:line whose behaviour it restates
+ * (paths relative to the Souvadra/dakc source tree).  Pins (what checks each
+ * function against something other than its own formula) are listed in
+ * tests/test_oracle.py:
+ *   dko_char2base      exhaustive 256-byte golden vector extracted from the
+ *                      reference's table (tests/golden/char2base_lut.bin)
+ *   dko_extract        brute-force substring enumeration in Python
+ *   dko_murmur64a_*    the general-length public MurmurHash64A written
+ *                      independently in Python; len=8 must equal the
+ *                      reference's single-block form
+ *   dko_count          collections.Counter + sorted() on tiny inputs, golden
+ *                      fixtures, bin-union == P=1 table, canonical identity
+ */
+#ifndef DAKC_ORACLE_H
+#define DAKC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: same numbering as include/dakc.h */
+#define DKO_OK 0
+#define DKO_E_INVAL 1
+#define DKO_E_FORMAT 2
+#define DKO_E_NOMEM 3
+#define DKO_E_TOOBIG 4
+
+/* kcounter.cpp:52 -- the fixed owner-hash seed */
+#define DKO_OWNER_SEED 0x9E3779B97F4A7C15ULL
+
+/* common.hpp:72-92: ASCII -> 2-bit code (C0 A1 T2 G3), 0xF0 for M/m, 0xFF otherwise */
+uint8_t dko_char2base(unsigned char c);
+
+/* kcounter.cpp:29-48: MurmurHash64A of one 8-byte key */
+uint64_t dko_murmur64a_u64(uint64_t key, uint64_t seed);
+
+/* owner hash for a 2x64-bit key: MurmurHash64A over 16 little-endian bytes
+ * (lo word then hi word).  Build decision; no reference counterpart. */
+uint64_t dko_murmur64a_u128(uint64_t lo, uint64_t hi, uint64_t seed);
+
+/* Extract every k-mer of a fixed-width record buffer (kcounter_funcs.cpp:25-111).
+ * Records are read_len bytes + '\n'.  Keys are written as key_words() words each
+ * (1 for k<=32: the value; 2 for k<=64: lo, hi).  Only k-mers whose owner ==
+ * owner_bin are kept when num_pes > 1 and owner_bin >= 0.
+ * Returns the k-mer count, or -(status) on error.  out may be NULL (count only). */
+int64_t dko_extract(const char *reads, size_t nbytes, int read_len, int k, int canonical,
+                    uint32_t num_pes, int32_t owner_bin, uint64_t *out, uint64_t cap);
+
+/* Full single-PE count (kcounter.cpp:247-414, HITTER=0): extract, sort ascending
+ * as unsigned, run-length reduce.  With num_pes > 1 and owner_bin == -1 the
+ * result is the concatenation of the num_pes owner tables in owner order and
+ * bin_offsets (num_pes+1 entries, may be NULL) gives their boundaries.
+ * *keys_out (d * key_words u64) and *counts_out (d u32) are malloc'd; free with
+ * dko_free.  Returns a status. */
+int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonical,
+              uint32_t num_pes, int32_t owner_bin, uint64_t **keys_out, uint32_t **counts_out,
+              uint64_t *d_out, uint64_t *n_out, uint64_t *bin_offsets);
+
+/* DAKC's P-PE shape on shared memory, for timing the CPU baseline:
+ * `threads` workers each parse 1/threads of the records (fqreader.cpp:43-52),
+ * scatter k-mers to threads owner bins by MurmurHash64A % threads
+ * (kcounter.cpp:50-54, 162-174), barrier, then each sorts and run-length
+ * reduces its own bin (kcounter.cpp:385-411).  Returns the total k-mer count,
+ * distinct count and an order-independent checksum of the (kmer, count)
+ * multiset (dko_table_checksum) so the result can be compared with any table.
+ * *elapsed_s is the wall time of the counting phases (file read excluded, as
+ * kcounter.cpp:262). */
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int threads,
+                 uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s);
+
+/* order-independent checksum of a table: sum over entries of mix(key words, count) */
+uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words);
+
+void dko_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

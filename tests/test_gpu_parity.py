@@ -1,0 +1,191 @@
+"""GPU parity: the HIP path, called through the C ABI, against the CPU oracle
+and the golden fixtures.  Integer work, so every comparison is bit-exact."""
+import ctypes
+import hashlib
+import json
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+from dakc_amd import DakcError, KmerCounter, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kc():
+    c = KmerCounter()
+    yield c
+    c.close()
+
+
+def assert_same(gpu, ora):
+    keys, counts, n, offs = ora
+    assert gpu.total == n
+    assert gpu.keys.shape == keys.shape, (gpu.keys.shape, keys.shape)
+    assert np.array_equal(gpu.keys, keys)
+    assert np.array_equal(gpu.counts, counts)
+    if offs is not None:
+        assert np.array_equal(gpu.bin_offsets, offs)
+
+
+def run_both(kc, data, L, k, canonical=False, P=1, b=-1):
+    g = kc.count(data, k=k, read_len=L, canonical=canonical, num_pes=P, owner_bin=b)
+    o = oracle.count(data, L, k, canonical, P, b)
+    assert_same(g, o)
+    return g
+
+
+def test_golden_fixtures(kc, golden_dir):
+    exp = json.loads((golden_dir / "expected.json").read_text())
+    for name, e in exp.items():
+        data = (golden_dir / e["reads"]).read_bytes()
+        t = kc.count(data, k=e["k"], read_len=e["read_len"], canonical=e["canonical"],
+                     num_pes=e["num_pes"], owner_bin=e["owner_bin"])
+        blob = t.serialize()
+        assert hashlib.sha256(blob).hexdigest() == e["sha256"], name
+        assert t.total == e["total"] and len(t) == e["distinct"], name
+        if "bin_offsets" in e:
+            assert t.bin_offsets.tolist() == e["bin_offsets"], name
+
+
+def messy_reads(seed, nreads, L):
+    rng = random.Random(seed)
+    g = "".join(rng.choice("ACGT") for _ in range(5000))
+    out = []
+    for _ in range(nreads):
+        ln = rng.randint(0, L)
+        st = rng.randint(0, len(g) - L)
+        s = list(g[st:st + ln])
+        for i in range(ln):
+            u = rng.random()
+            if u < 0.02:
+                s[i] = rng.choice("acgtNRm \r\x01\xff")
+        out.append("".join(s) + "M" * (L - ln) + "\n")
+    return "".join(out).encode("latin-1")
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 16, 21, 31, 32, 33, 40, 55, 63, 64])
+@pytest.mark.parametrize("canonical", [False, True])
+def test_messy_reads_all_k(kc, k, canonical):
+    L = 150 if k <= 32 else 250
+    run_both(kc, messy_reads(k * 7 + canonical, 700, L), L, k, canonical)
+
+
+@pytest.mark.parametrize("L", [1, 7, 31, 32, 33, 63, 64, 65, 100, 257, 1000, 4096])
+def test_read_widths(kc, L):
+    data = messy_reads(L, 200 if L < 1000 else 20, L)
+    for k in sorted({1, min(L, 21), min(L, 31)}):
+        run_both(kc, data, L, k)
+
+
+def test_iid_reads_100k(kc):
+    p = _lib.gen_params(42, 100_000, 150)
+    data = _lib.generate_reads(p)
+    t = run_both(kc, data, 150, 31)
+    assert t.total == 100_000 * 120
+
+
+def test_genome_reads_repeats(kc):
+    p = _lib.gen_params(7, 200_000, 150, genome_len=1_000_000, sub_rate=0.005)
+    data = _lib.generate_reads(p)
+    t = run_both(kc, data, 150, 31)
+    assert t.counts.max() > 10
+    run_both(kc, data, 150, 31, canonical=True)
+    run_both(kc, data, 150, 21)
+
+
+def test_k63_owner_bins(kc):
+    p = _lib.gen_params(3, 20_000, 250, genome_len=400_000, sub_rate=0.01, n_rate=0.001)
+    data = _lib.generate_reads(p)
+    run_both(kc, data, 250, 63, P=64, b=0)
+    run_both(kc, data, 250, 63, P=64, b=-1)
+    run_both(kc, data, 250, 63)
+
+
+def test_owner_bins_k31(kc):
+    p = _lib.gen_params(4, 50_000, 150, genome_len=500_000, sub_rate=0.005)
+    data = _lib.generate_reads(p)
+    for P, b in [(64, 0), (64, 63), (64, -1), (7, -1), (256, -1), (1000, 3)]:
+        run_both(kc, data, 150, 31, P=P, b=b)
+
+
+def test_runs_longer_than_a_tile(kc):
+    # identical reads: every k-mer's run spans many tiles (gallop path)
+    read = ("ACGTTGCA" * 19)[:150]
+    data = ((read + "\n") * 3000).encode()
+    t = run_both(kc, data, 150, 31)
+    assert t.counts.min() >= 3000
+    polya = (("A" * 150 + "\n") * 2000).encode()
+    t = run_both(kc, polya, 150, 31)
+    assert len(t) == 1 and t.counts[0] == 2000 * 120
+
+
+def test_empty_and_short(kc):
+    t = kc.count(b"", k=31, read_len=150)
+    assert len(t) == 0 and t.total == 0
+    t = kc.count(b"ACGT\n" * 10, k=31, read_len=4)   # k > read_len
+    assert len(t) == 0 and t.total == 0
+    t = kc.count((b"N" * 150 + b"\n") * 10, k=31, read_len=150)
+    assert len(t) == 0
+
+
+def test_format_errors(kc):
+    good = (b"ACGT" * 37 + b"AC\n") * 20
+    assert kc.count(good, k=31, read_len=150).total == 20 * 120
+    assert kc.count(good[:-1], k=31, read_len=150).total == 20 * 120
+    for bad in (good[:-2], good[:150] + b"X" + good[151:], b"\x00" + good[1:],
+                good[:151] + b"\x00" + good[152:]):
+        with pytest.raises(DakcError) as e:
+            kc.count(bad, k=31, read_len=150)
+        assert e.value.status == 2
+    for kw in ({"k": 0}, {"k": 65}, {"read_len": 0}, {"num_pes": 0}, {"num_pes": 4, "owner_bin": 4}):
+        with pytest.raises(DakcError) as e:
+            kc.count(good, **{"k": 31, "read_len": 150, **kw})
+        assert e.value.status == 1
+    # a later call still works
+    assert kc.count(good, k=31, read_len=150).total == 20 * 120
+
+
+def test_device_generator_matches_host(kc):
+    for p in (_lib.gen_params(11, 20_000, 150, genome_len=100_000, sub_rate=0.005),
+              _lib.gen_params(12, 5_000, 250, min_len=100, sub_rate=0.005, n_rate=0.001),
+              _lib.gen_params(13, 3_001, 151)):
+        host = _lib.generate_reads(p)
+        dptr = kc.device_alloc(host.size)
+        try:
+            kc.generate_device(p, dptr, host.size)
+            dev = kc.d2h(dptr, host.size)
+        finally:
+            kc.device_free(dptr)
+        assert np.array_equal(host, dev)
+
+
+def test_device_resident_input(kc):
+    p = _lib.gen_params(21, 30_000, 150, genome_len=300_000, sub_rate=0.005)
+    host = _lib.generate_reads(p)
+    want = kc.count(host)
+    dptr = kc.device_alloc(host.size)
+    try:
+        kc.generate_device(p, dptr, host.size)
+        t = kc.count_device(dptr, host.size)
+        assert t.on_device == 1 and t.n == len(want)
+        kptr = ctypes.cast(t.keys, ctypes.c_void_p).value
+        cptr = ctypes.cast(t.counts, ctypes.c_void_p).value
+        keys = kc.d2h(kptr, t.n * 8, np.uint64)
+        counts = kc.d2h(cptr, t.n * 4, np.uint32)
+        assert np.array_equal(keys, want.keys[:, 0]) and np.array_equal(counts, want.counts)
+        kc._libh.dakc_table_free(ctypes.byref(t))
+    finally:
+        kc.device_free(dptr)
+
+
+def test_repeat_runs_identical(kc):
+    p = _lib.gen_params(5, 60_000, 150, genome_len=200_000, sub_rate=0.005)
+    data = _lib.generate_reads(p)
+    a = kc.count(data)
+    for _ in range(3):
+        b = kc.count(data)
+        assert np.array_equal(a.keys, b.keys) and np.array_equal(a.counts, b.counts)

@@ -67,6 +67,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--algo", default="auto", choices=["auto", "lsd"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU's count; --gpus {a.gpus} is not supported in this build "
@@ -76,7 +77,8 @@ def main() -> int:
     from dakc_amd import KmerCounter, _lib
 
     w = WORKLOADS[a.workload]
-    kc = KmerCounter(k=w["k"], read_len=w["read_len"])
+    kc = KmerCounter(k=w["k"], read_len=w["read_len"],
+                     algo=_lib.ALGO_LSD if a.algo == "lsd" else _lib.ALGO_AUTO)
     p = _lib.gen_params(w["seed"], w["num_reads"], w["read_len"], genome_len=w["genome_len"],
                         sub_rate=w["sub_rate"])
     nbytes = w["num_reads"] * (w["read_len"] + 1)
@@ -89,39 +91,55 @@ def main() -> int:
         t = kc.count_device(dreads, nbytes)
         print(f"[bench] warmup {i}: {t.total} k-mers, {t.n} distinct", file=sys.stderr, flush=True)
     n_kmers = d = 0
+    acc = {"hist": 0.0, "pass0": 0.0, "key_passes": 0.0, "rle": 0.0, "part2": 0.0, "final": 0.0}
     sweep_sum = np.zeros(18)
-    hist_ms = pass0_ms = rle_ms = 0.0
-    npasses = 0
+    npasses = path = reruns = 0
     t0 = time.perf_counter()
     for _ in range(a.steps):
         t = kc.count_device(dreads, nbytes)
         s = kc.last_stats()
         n_kmers, d = int(t.total), int(t.n)
         sweep_sum += np.array(s.sweep_ms[:])
-        hist_ms += s.hist_ms
-        pass0_ms += s.pass0_ms
-        rle_ms += s.rle_ms
-        npasses = int(s.npasses)
+        acc["hist"] += s.hist_ms
+        acc["pass0"] += s.pass0_ms
+        acc["key_passes"] += s.passes_ms
+        acc["rle"] += s.rle_ms
+        acc["part2"] += s.part2_ms
+        acc["final"] += s.final_ms
+        npasses, path, reruns = int(s.npasses), int(s.path), int(s.msd_reruns)
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / a.steps
     value = n_kmers / (ms_per_step / 1e3)
+    ms = {k: v / a.steps for k, v in acc.items()}
 
-    # dominant kernel: the key-to-key LSD pass (k_sweep<from_reads=false>),
-    # algorithmic bytes = n keys read + n keys written, 8 B each
-    key_pass_ms = sweep_sum[1:npasses] / a.steps
-    avg_pass_ms = float(key_pass_ms.mean()) if npasses > 1 else float("nan")
-    alg_bytes = 16.0 * n_kmers
-    achieved = alg_bytes / (avg_pass_ms / 1e3) / 1e9
-    kb = 8
-    in_bytes = nbytes
-    b_alg = 2 * in_bytes + n_kmers * kb + 2 * (npasses - 1) * n_kmers * kb + n_kmers * kb + d * (kb + 4)
+    # algorithmic bytes per launch of each kernel (compulsory reads + writes):
+    # in = record bytes, n = k-mers, d = distinct, w = key bytes, c = 4 (count)
+    kb, inb = 8 * (1 if w["k"] <= 32 else 2), nbytes
+    kernels = {}
+    kernels["k_hist_reads"] = (ms["hist"], inb)
+    kernels["k_sweep<from_reads>"] = (ms["pass0"], inb + n_kmers * kb)
+    if path == 2:
+        kernels["k_part2"] = (ms["part2"], 2 * n_kmers * kb)
+        kernels["k_final"] = (ms["final"], n_kmers * kb + d * (kb + 4))
+        b_alg = 2 * inb + 4 * n_kmers * kb + d * (kb + 4)
+    else:
+        key_pass_ms = sweep_sum[1:npasses] / a.steps
+        kernels["k_sweep<keys>"] = (float(key_pass_ms.mean()) if npasses > 1 else 0.0, 2 * n_kmers * kb)
+        kernels["k_rle"] = (ms["rle"], n_kmers * kb + d * (kb + 4))
+        b_alg = 2 * inb + n_kmers * kb + 2 * (npasses - 1) * n_kmers * kb + n_kmers * kb + d * (kb + 4)
+    total_kernel_ms = {name: (t_ms * (npasses - 1) if name == "k_sweep<keys>" else t_ms)
+                       for name, (t_ms, _) in kernels.items()}
+    dom = max(total_kernel_ms, key=total_kernel_ms.get)
+    dom_ms, dom_bytes = kernels[dom]
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     stages = {
-        "hist_ms": hist_ms / a.steps, "pass0_ms": pass0_ms / a.steps,
-        "key_passes_ms": float(key_pass_ms.sum()), "rle_ms": rle_ms / a.steps,
-        "npasses": npasses, "distinct": d,
+        "path": "msd" if path == 2 else "lsd", "msd_reruns": reruns, "distinct": d,
+        "kernels": {name: {"avg_ms": t_ms, "alg_bytes": nb_,
+                           "GBps": nb_ / (t_ms / 1e3) / 1e9 if t_ms > 0 else None}
+                    for name, (t_ms, nb_) in kernels.items()},
         "whole_job_GBps_alg": b_alg / (ms_per_step / 1e3) / 1e9,
         "B_alg_per_kmer": b_alg / max(n_kmers, 1),
-        "B_ref_per_kmer": (in_bytes + n_kmers * kb + 8 * n_kmers * kb) / max(n_kmers, 1),
+        "B_ref_per_kmer": (inb + n_kmers * kb + 8 * n_kmers * kb) / max(n_kmers, 1),
     }
     out = {
         "metric": "k-mers counted/s",
@@ -141,9 +159,8 @@ def main() -> int:
                                f"k={w['k']}", "k": w["k"], "kmers_per_step": n_kmers,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep (key-to-key LSD pass)", "avg_ms": avg_pass_ms,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
     }
     kc.device_free(dreads)

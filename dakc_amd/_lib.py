@@ -21,6 +21,7 @@ STATUS_NAMES = {
     9: "DAKC_E_PENDING", 10: "DAKC_E_CANCELLED",
 }
 FMT_FIXED = 0
+ALGO_AUTO, ALGO_LSD, ALGO_MSD = 0, 1, 2
 
 # every symbol include/dakc.h declares
 EXPORTS = [
@@ -58,7 +59,7 @@ class Table(C.Structure):
 
 
 class Opts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_int32), ("algo", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -66,6 +67,8 @@ class Stats(C.Structure):
         ("hist_ms", C.c_double), ("pass0_ms", C.c_double), ("passes_ms", C.c_double),
         ("rle_ms", C.c_double), ("total_ms", C.c_double), ("npasses", C.c_uint32),
         ("tiles0", C.c_uint32), ("n", C.c_uint64), ("d", C.c_uint64), ("sweep_ms", C.c_double * 18),
+        ("path", C.c_uint32), ("msd_reruns", C.c_uint32), ("part2_ms", C.c_double),
+        ("final_ms", C.c_double),
     ]
 
 

@@ -80,7 +80,7 @@ void DevBuf::release() {
 
 JobWs::~JobWs() {
   for (auto *b : {&hist, &dstart, &ndev, &err, &tickets, &flags, &rle_flags, &keys_a, &keys_b,
-                  &counts, &binoff, &reads})
+                  &counts, &binoff, &reads, &msd_ts, &msd_off})
     b->release();
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
@@ -125,6 +125,7 @@ size_t job_device_bytes(const Plan &pl) {
   b += (size_t)std::max<uint64_t>(pl.tiles0, pl.tiles_keys) * 256 * 8 + (size_t)pl.tiles_keys * 8;
   b += 2 * (size_t)pl.nmax * kb + (size_t)pl.nmax * 4;
   b += (size_t)pl.nbytes + 256;
+  if (pl.msd) b += (size_t)pl.tiles2 * (((size_t)1 << pl.b2) + 1) * 2 + 4096 + (size_t)pl.nsb * 8;
   return b;
 }
 
@@ -136,7 +137,8 @@ int JobWs::reserve(const Plan &pl, bool need_reads) {
   if ((s = ndev.ensure(64, stream, false))) return s;
   if ((s = err.ensure(64, stream, false))) return s;
   if ((s = tickets.ensure((MAX_PASSES + 2) * 4, stream, false))) return s;
-  const size_t ft = std::max<uint64_t>(pl.tiles0, pl.tiles_keys);
+  size_t ft = std::max<uint64_t>(pl.tiles0, pl.tiles_keys);
+  if (pl.msd) ft = std::max<size_t>(ft, ((size_t)pl.nsb + 255) / 256);
   const size_t fl_before = flags.cap, rf_before = rle_flags.cap;
   if ((s = flags.ensure(std::max<size_t>(ft, 1) * 256 * 8, stream, true))) return s;
   if ((s = rle_flags.ensure(std::max<size_t>(pl.tiles_keys, 1) * 8, stream, true))) return s;
@@ -151,6 +153,12 @@ int JobWs::reserve(const Plan &pl, bool need_reads) {
   if ((s = counts.ensure(std::max<size_t>(pl.nmax, 1) * 4 + 256, stream, false))) return s;
   if ((s = binoff.ensure(((size_t)pl.num_pes + 1) * 8 + 64, stream, false))) return s;
   if (need_reads && (s = reads.ensure(pl.nbytes + 256, stream, false))) return s;
+  if (pl.msd) {
+    if ((s = msd_ts.ensure(260 * 4, stream, false))) return s;
+    if ((s = msd_off.ensure(std::max<size_t>(pl.tiles2, 1) * (((size_t)1 << pl.b2) + 1) * 2 + 256,
+                            stream, false)))
+      return s;
+  }
   return DAKC_OK;
 }
 
@@ -208,6 +216,36 @@ int make_plan(const dakc_job *job, Plan *pl) {
   return DAKC_OK;
 }
 
+// MSD geometry: first-level bits b1 (<= 8, fewer when owner bins share the
+// 256 first-level buckets), second-level b2 (<= 11), chosen so an average
+// sub-bucket holds at most 60% of what one workgroup sorts in LDS; b3 (<= 11)
+// splits a sub-bucket inside LDS into bins short enough to order by counting.
+int plan_choose_path(Plan *pl, int algo) {
+  Plan &p = *pl;
+  p.msd = 0;
+  if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
+  const uint32_t bits = 2 * (uint32_t)p.k;
+  const uint64_t cap = (uint64_t)msd_tile(p.kw);
+  const uint64_t target = cap * 3 / 5;
+  uint32_t b12 = 0;
+  while ((p.nmax >> b12) > target) ++b12;
+  uint32_t owners = p.owner_digit ? p.num_pes : 1;
+  if (owners > 256) return 0;
+  uint32_t b1max = 0;
+  while ((owners << (b1max + 1)) <= 256 && b1max < 8) ++b1max;
+  const uint32_t b1 = std::min(std::min(b1max, b12), bits);
+  const uint32_t b2 = std::min(std::min<uint32_t>(11, b12 - b1), bits - b1);
+  if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
+  p.b1 = b1;
+  p.b2 = b2;
+  p.b3 = std::min<uint32_t>(11, bits - b1 - b2);
+  p.nb1 = owners << b1;
+  p.nsb = p.nb1 << b2;
+  p.tiles2 = (uint32_t)((p.nmax + cap - 1) / cap) + p.nb1;
+  p.msd = 1;
+  return 1;
+}
+
 ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
   ReadsDesc rd;
   rd.reads = dreads;
@@ -229,33 +267,78 @@ ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
 
 // Enqueue the whole count of one job on ws.stream.  The sorted table ends in
 // *table_keys (one of the two key buffers) and ws.counts.
+static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd, int num_cus, bool events,
+                       void **table_keys) {
+  hipStream_t st = ws.stream;
+  Workspace w = ws.view();
+  const uint32_t bits = 2 * (uint32_t)pl.k;
+  DigitSpec d1 = {DIG_MSD, bits - pl.b1, pl.b1, pl.owner_digit ? pl.num_pes : 1u};
+  HistSpecs hs;
+  hs.n = 1;
+  hs.d[0] = d1;
+  launch_fill_u64(w.hist, 0, 256, st);
+  launch_fill_u64(w.n_dev, 0, 2, st);
+  launch_fill_u32(w.err, 0, 1, st);
+  launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
+  if (events) hipEventRecord(ws.ev[0], st);
+  const int grid = (int)std::min<uint64_t>(pl.tiles0, (uint64_t)num_cus * 4);
+  launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
+  launch_scan_digits(1, w, st);
+  if (events) hipEventRecord(ws.ev[1], st);
+  launch_sweep(pl.kw, true, rd, nullptr, w.keys_a, d1, w.digit_start, w, 0, ws.next_epoch(),
+               pl.tiles0, st);
+  if (events) hipEventRecord(ws.ev[2], st);
+  MsdArgs m;
+  m.nb1 = pl.nb1;
+  m.b2 = pl.b2;
+  m.d2 = {DIG_BITS, bits - pl.b1 - pl.b2, pl.b2, 1};
+  m.d3 = {DIG_BITS, bits - pl.b1 - pl.b2 - pl.b3, pl.b3, 1};
+  m.tile_start = (uint32_t *)ws.msd_ts.p;
+  m.offsets = (uint16_t *)ws.msd_off.p;
+  launch_bucket_tiles(pl.kw, m, w, st);
+  launch_part2(pl.kw, w.keys_a, w.keys_b, m, w, pl.tiles2, st);
+  if (events) hipEventRecord(ws.ev[3], st);
+  launch_final(pl.kw, w.keys_b, w.keys_a, m, w, ws.next_epoch(), st);
+  if (pl.owner_digit) launch_bin_offsets(pl.kw, w.keys_a, pl.num_pes, w, st);
+  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
+  *table_keys = w.keys_a;
+  return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
 int enqueue_count(JobWs &ws, const Plan &pl, const uint8_t *dreads, int num_cus, bool events,
                   void **table_keys) {
   hipStream_t st = ws.stream;
   Workspace w = ws.view();
   const ReadsDesc rd = plan_reads(pl, dreads);
+  if (pl.msd) return enqueue_msd(ws, pl, rd, num_cus, events, table_keys);
   launch_fill_u64(w.hist, 0, (uint64_t)pl.npasses * 256, st);
   launch_fill_u64(w.n_dev, 0, 2, st);
   launch_fill_u32(w.err, 0, 1, st);
   launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
   if (events) hipEventRecord(ws.ev[0], st);
+  auto pass_digit = [&](int p) -> DigitSpec {
+    if (pl.owner_digit && p == pl.npasses - 1) return DigitSpec{DIG_OWNER, 0, 8, pl.num_pes};
+    return DigitSpec{DIG_BITS, 8u * (uint32_t)p, 8, 1};
+  };
   if (pl.nrec > 0) {
+    HistSpecs hs;
+    hs.n = pl.npasses;
+    for (int p = 0; p < pl.npasses; ++p) hs.d[p] = pass_digit(p);
     const int grid = (int)std::min<uint64_t>(pl.tiles0, (uint64_t)num_cus * 4);
-    launch_hist_reads(pl.kw, rd, pl.ndig, pl.owner_digit, w, pl.tiles0, grid, st);
+    launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
     launch_scan_digits(pl.npasses, w, st);
   }
   if (events) hipEventRecord(ws.ev[1], st);
   void *cur = w.keys_a, *other = w.keys_b;
   if (pl.nmax > 0) {
     for (int p = 0; p < pl.npasses; ++p) {
-      const bool owner_pass = pl.owner_digit && p == pl.npasses - 1;
       const uint32_t ep = ws.next_epoch();
       if (p == 0) {
-        launch_sweep(pl.kw, true, rd, nullptr, cur, 0, false, pl.num_pes, w.digit_start, w, 0, ep,
+        launch_sweep(pl.kw, true, rd, nullptr, cur, pass_digit(0), w.digit_start, w, 0, ep,
                      pl.tiles0, st);
       } else {
-        launch_sweep(pl.kw, false, rd, cur, other, 8 * p, owner_pass, pl.num_pes,
-                     w.digit_start + 256 * p, w, p, ep, pl.tiles_keys, st);
+        launch_sweep(pl.kw, false, rd, cur, other, pass_digit(p), w.digit_start + 256 * p, w, p, ep,
+                     pl.tiles_keys, st);
         std::swap(cur, other);
       }
       if (events) hipEventRecord(ws.ev[2 + p], st);
@@ -275,7 +358,8 @@ int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d) {
   uint32_t e = 0;
   if (hipMemcpy(nd, ws.ndev.p, 16, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
   if (hipMemcpy(&e, ws.err.p, 4, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
-  if (e) return DAKC_E_FORMAT;
+  if (e & (ERR_SEPARATOR | ERR_NUL_RECORD)) return DAKC_E_FORMAT;
+  if (e & ERR_OVERFLOW) return STATUS_MSD_OVERFLOW;
   if (pl.nmax == 0) nd[1] = 0;
   *n = nd[0];
   *d = nd[1];
@@ -333,39 +417,24 @@ extern "C" int dakc_ctx_destroy(dakc_ctx *c) {
 
 // ----------------------------------------------------------- plain count --
 
-static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool to_host) {
-  if (!c || !job || !out) return DAKC_E_INVAL;
-  std::memset(out, 0, sizeof(*out));
-  hipSetDevice(c->device);
-  Plan pl;
-  int s = make_plan(job, &pl);
-  if (s) return s;
-  JobWs &ws = c->ws;
-  if ((s = ws.reserve(pl, !job->reads_on_device))) return s;
-  const uint8_t *dreads = (const uint8_t *)job->reads;
-  if (!job->reads_on_device && pl.nbytes) {
-    if (hipMemcpyAsync(ws.reads.p, job->reads, pl.nbytes, hipMemcpyHostToDevice, ws.stream) !=
-        hipSuccess)
-      return DAKC_E_DEVICE;
-    dreads = (const uint8_t *)ws.reads.p;
-  }
-  void *tkeys = nullptr;
-  if ((s = enqueue_count(ws, pl, dreads, c->num_cus, true, &tkeys))) return s;
-  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
-  uint64_t n = 0, d = 0;
-  if ((s = finish_count(ws, pl, &n, &d))) return s;
-
-  // stage timings
-  dakc_stats &st = c->last;
+static void fill_stats(dakc_stats &st, JobWs &ws, const Plan &pl, uint64_t n, uint64_t d) {
   std::memset(&st, 0, sizeof(st));
   float ms = 0;
-  st.npasses = pl.nmax ? (uint32_t)pl.npasses : 0;
+  st.npasses = pl.nmax ? (uint32_t)(pl.msd ? 1 : pl.npasses) : 0;
   st.tiles0 = pl.tiles0;
   st.n = n;
   st.d = d;
+  st.path = pl.msd ? 2 : 1;
   hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]);
   st.hist_ms = ms;
-  if (pl.nmax) {
+  if (pl.nmax && pl.msd) {
+    hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);
+    st.pass0_ms = st.sweep_ms[0] = ms;
+    hipEventElapsedTime(&ms, ws.ev[2], ws.ev[3]);
+    st.part2_ms = ms;
+    hipEventElapsedTime(&ms, ws.ev[3], ws.ev[2 + MAX_PASSES]);
+    st.final_ms = ms;
+  } else if (pl.nmax) {
     hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);
     st.pass0_ms = ms;
     st.sweep_ms[0] = ms;
@@ -379,6 +448,42 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   }
   hipEventElapsedTime(&ms, ws.ev[0], ws.ev[2 + MAX_PASSES]);
   st.total_ms = ms;
+}
+
+static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool to_host) {
+  if (!c || !job || !out) return DAKC_E_INVAL;
+  std::memset(out, 0, sizeof(*out));
+  hipSetDevice(c->device);
+  Plan pl;
+  int s = make_plan(job, &pl);
+  if (s) return s;
+  plan_choose_path(&pl, c->opts.algo);
+  JobWs &ws = c->ws;
+  const uint8_t *dreads = (const uint8_t *)job->reads;
+  void *tkeys = nullptr;
+  uint64_t n = 0, d = 0;
+  uint32_t reruns = 0;
+  for (;;) {
+    if ((s = ws.reserve(pl, !job->reads_on_device))) return s;
+    if (!job->reads_on_device && pl.nbytes && dreads == (const uint8_t *)job->reads) {
+      if (hipMemcpyAsync(ws.reads.p, job->reads, pl.nbytes, hipMemcpyHostToDevice, ws.stream) !=
+          hipSuccess)
+        return DAKC_E_DEVICE;
+      dreads = (const uint8_t *)ws.reads.p;
+    }
+    if ((s = enqueue_count(ws, pl, dreads, c->num_cus, true, &tkeys))) return s;
+    if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+    s = finish_count(ws, pl, &n, &d);
+    if (s == STATUS_MSD_OVERFLOW && pl.msd) {  // skewed input: count it on the LSD path
+      plan_choose_path(&pl, DAKC_ALGO_LSD);
+      ++reruns;
+      continue;
+    }
+    if (s) return s;
+    break;
+  }
+  fill_stats(c->last, ws, pl, n, d);
+  c->last.msd_reruns = reruns;
 
   out->n = d;
   out->total = n;

@@ -34,11 +34,21 @@ struct Plan {
   int32_t owner_bin = -1;
   int ndig = 0, owner_digit = 0, npasses = 0;
   uint32_t canonical = 0, last_has_nl = 1;
+  // MSD path (msd = 1): first-level bits b1, second-level b2, in-LDS b3
+  int msd = 0;
+  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nsb = 1, tiles2 = 0;
 };
+
+// internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
+constexpr int STATUS_MSD_OVERFLOW = -100;
+
+// path choice (dakc_opts.algo)
+int plan_choose_path(Plan *pl, int algo);
 
 // Device workspace of one in-flight job (one per served slot).
 struct JobWs {
-  DevBuf hist, dstart, ndev, err, tickets, flags, rle_flags, keys_a, keys_b, counts, binoff, reads;
+  DevBuf hist, dstart, ndev, err, tickets, flags, rle_flags, keys_a, keys_b, counts, binoff, reads,
+      msd_ts, msd_off;
   hipStream_t stream = nullptr;
   hipEvent_t ev[MAX_PASSES + 4] = {};
   uint32_t epoch = 0;

@@ -68,6 +68,19 @@ DK_DEV uint32_t owner_of(const KeyT &x, uint32_t num_pes) {
   return (uint32_t)(murmur64a(x, OWNER_SEED) % (uint64_t)num_pes);
 }
 
+// bits [shift, shift + nbits) of the key value, nbits <= 32
+DK_DEV uint32_t key_bits(uint64_t x, uint32_t shift, uint32_t nbits) {
+  if (!nbits) return 0;
+  return (uint32_t)(x >> shift) & (0xFFFFFFFFu >> (32 - nbits));
+}
+DK_DEV uint32_t key_bits(const K2 &x, uint32_t shift, uint32_t nbits) {
+  if (!nbits) return 0;
+  uint64_t v;
+  if (shift >= 64) v = x.hi >> (shift - 64);
+  else v = shift ? (x.lo >> shift) | (x.hi << (64 - shift)) : x.lo;
+  return (uint32_t)v & (0xFFFFFFFFu >> (32 - nbits));
+}
+
 // reverse the order of the 32 2-bit groups of a word
 DK_DEV uint64_t rev2(uint64_t x) {
   x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);

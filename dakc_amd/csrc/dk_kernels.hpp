@@ -26,7 +26,20 @@ struct ReadsDesc {
 };
 
 // error bits reported through the job's err word
-enum : uint32_t { ERR_SEPARATOR = 1u, ERR_NUL_RECORD = 2u };
+enum : uint32_t { ERR_SEPARATOR = 1u, ERR_NUL_RECORD = 2u, ERR_OVERFLOW = 4u };
+
+// Which digit of a key a pass sorts or partitions by.
+//   DIG_BITS : bits [shift, shift + nbits) of the 2k-bit k-mer value
+//   DIG_OWNER: MurmurHash64A owner % num_pes
+//   DIG_MSD  : owner * 2^nbits + bits [shift, shift + nbits) (owner part only
+//              when num_pes > 1), so buckets come out in (owner, prefix) order
+enum : uint32_t { DIG_BITS = 0, DIG_OWNER = 1, DIG_MSD = 2 };
+struct DigitSpec {
+  uint32_t mode;
+  uint32_t shift;
+  uint32_t nbits;
+  uint32_t num_pes;
+};
 
 struct Workspace {
   uint64_t *hist;         // (MAX_PASSES) x 256 digit counts
@@ -42,27 +55,54 @@ struct Workspace {
 };
 
 constexpr int MAX_PASSES = 18;  // 16 digits of a 128-bit key + owner digit + spare
+#define MAX_PASSES_DECL 18
 
 // Tile geometry of the radix passes (keys per tile = NT * ITEMS)
 constexpr int SWEEP_NT = 512;
 constexpr int SWEEP_ITEMS_1 = 15;  // 1 x u64 keys: 7680 per tile
 constexpr int SWEEP_ITEMS_2 = 8;   // 2 x u64 keys: 4096 per tile
 constexpr int RAW_CAP = 16384;     // LDS bytes for one tile of raw records
-constexpr int MAX_READ_LEN = 12000;
+constexpr int MAX_READ_LEN = 4096;  // keeps L - k + 1 <= keys per tile
 
 inline int tile_keys(int kw) { return SWEEP_NT * (kw == 1 ? SWEEP_ITEMS_1 : SWEEP_ITEMS_2); }
+
+// MSD second level and in-LDS count: one 1024-thread workgroup per CU holds
+// a whole tile / sub-bucket (15360 u64 or 8192 2x u64 keys) in LDS
+constexpr int MSD_NT = 1024;
+constexpr int MSD_ITEMS_1 = 15;
+constexpr int MSD_ITEMS_2 = 8;
+inline int msd_tile(int kw) { return MSD_NT * (kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2); }
 
 // Launchers (all asynchronous on `st`).
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st);
-void launch_hist_reads(int kw, const ReadsDesc &rd, int ndig, int owner_digit, const Workspace &ws,
+struct HistSpecs {
+  DigitSpec d[MAX_PASSES_DECL];
+  int n;
+};
+void launch_hist_reads(int kw, const ReadsDesc &rd, const HistSpecs &hs, const Workspace &ws,
                        uint32_t ntiles, int grid, hipStream_t st);
 void launch_scan_digits(int npasses, const Workspace &ws, hipStream_t st);
 // one LSD pass; pass 0 reads the records, later passes read keys
 void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, void *out,
-                  uint32_t shift, bool owner_pass, uint32_t num_pes, const uint64_t *digit_start,
-                  const Workspace &ws, uint32_t ticket_slot, uint32_t epoch, uint32_t grid,
-                  hipStream_t st);
+                  const DigitSpec &dig, const uint64_t *digit_start, const Workspace &ws,
+                  uint32_t ticket_slot, uint32_t epoch, uint32_t grid, hipStream_t st);
+
+// MSD path: after an MSD sweep from the records into nb1 buckets, split every
+// bucket's tiles locally by a second digit, then count each (bucket, digit)
+// sub-bucket inside LDS.
+struct MsdArgs {
+  uint32_t nb1;         // first-level buckets
+  uint32_t b2;          // second-level digit bits
+  DigitSpec d2, d3;     // second-level digit, in-LDS partition digit
+  uint32_t *tile_start; // nb1 + 1 prefix of tiles per bucket
+  uint16_t *offsets;    // per bucket b: [2^b2 + 1][tiles of b] exclusive segment starts
+};
+void launch_bucket_tiles(int kw, const MsdArgs &m, const Workspace &ws, hipStream_t st);
+void launch_part2(int kw, const void *in, void *out, const MsdArgs &m, const Workspace &ws,
+                  uint32_t grid, hipStream_t st);
+void launch_final(int kw, const void *in, void *out_keys, const MsdArgs &m, const Workspace &ws,
+                  uint32_t epoch, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,

@@ -188,12 +188,22 @@ DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, ExtractSmem &sm, Ke
 
 // ------------------------------------------------------------- histogram --
 
+template <typename KeyT>
+DK_DEV uint32_t digit_eval(const DigitSpec &ds, const KeyT &key) {
+  if (ds.mode == DIG_OWNER) return owner_of(key, ds.num_pes);
+  uint32_t v = key_bits(key, ds.shift, ds.nbits);
+  if (ds.mode == DIG_MSD && ds.num_pes > 1) v += owner_of(key, ds.num_pes) << ds.nbits;
+  return v;
+}
+
+// Digit histograms (<= 256 bins each) of every k-mer of the records; also the
+// record-format checks.  Persistent grid: LDS counts, one flush per group.
 template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_hist_reads(ReadsDesc rd, int ndig, int owner_digit,
-                                                   uint32_t ntiles, uint64_t *hist, uint32_t *err) {
+__global__ __launch_bounds__(NT) void k_hist_reads(ReadsDesc rd, HistSpecs hs, uint32_t ntiles,
+                                                   uint64_t *hist, uint32_t *err) {
   __shared__ ExtractSmem ex;
   __shared__ uint32_t s_h[MAX_PASSES * 256];
-  const int nh = (ndig + owner_digit) * 256;
+  const int nh = hs.n * 256;
   for (int i = threadIdx.x; i < nh; i += NT) s_h[i] = 0;
   // (the first barrier inside extract_tile orders the clear before use)
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -203,8 +213,7 @@ __global__ __launch_bounds__(NT) void k_hist_reads(ReadsDesc rd, int ndig, int o
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if (!((vmask >> j) & 1)) continue;
-      for (int p = 0; p < ndig; ++p) atomicAdd(&s_h[p * 256 + key_digit(keys[j], 8 * p)], 1u);
-      if (owner_digit) atomicAdd(&s_h[ndig * 256 + owner_of(keys[j], rd.num_pes)], 1u);
+      for (int p = 0; p < hs.n; ++p) atomicAdd(&s_h[p * 256 + digit_eval(hs.d[p], keys[j])], 1u);
     }
     __syncthreads();
   }
@@ -245,9 +254,7 @@ struct SweepArgs {
   const uint64_t *digit_start;
   uint64_t *flags;
   uint32_t *ticket;
-  uint32_t shift;
-  uint32_t owner_pass;
-  uint32_t num_pes;
+  DigitSpec dig;
   uint32_t epoch;
 };
 
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
   for (int j = 0; j < ITEMS; ++j) {
     const bool v = (vmask >> j) & 1;
     uint32_t d = 0;
-    if (v) d = a.owner_pass ? owner_of(keys[j], a.num_pes) : key_digit(keys[j], a.shift);
+    if (v) d = digit_eval(a.dig, keys[j]);
     uint64_t peers = __ballot(v);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -486,6 +493,307 @@ __global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uin
   }
 }
 
+// ------------------------------------------------------------- MSD path --
+//
+// One GPU's counting without the LSD passes: an MSD sweep from the records
+// puts every k-mer in one of nb1 first-level buckets (top bits, and the owner
+// when all P bins are counted); k_part2 splits each bucket tile by the next b2
+// bits inside LDS and writes the tile back in digit order with its segment
+// starts; k_final gathers one (bucket, digit) sub-bucket from all tiles of its
+// bucket into LDS, sorts it there, run-length reduces it and writes its
+// (kmer, count) entries at the position the decoupled look-back gives.
+
+// exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = total
+template <int NT>
+DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_total) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t v[4], sum = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t i = tid * 4 + q;
+    v[q] = i < n ? s[i] : 0;
+    sum += v[q];
+  }
+  uint32_t pre = scan_block<NT>(sum, s_wave, s_total);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t i = tid * 4 + q;
+    if (i < n) s[i] = pre;
+    pre += v[q];
+  }
+  if (tid == 0) s[n] = *s_total;
+  __syncthreads();
+}
+
+// tiles per first-level bucket -> prefix
+__global__ __launch_bounds__(256) void k_bucket_tiles(const uint64_t *hist, uint32_t nb1,
+                                                      uint32_t tile2, uint32_t *tile_start) {
+  __shared__ uint32_t s[257];
+  const uint32_t b = threadIdx.x;
+  s[b] = b < nb1 ? (uint32_t)((hist[b] + tile2 - 1) / tile2) : 0;
+  if (b == 0) s[256] = 0;
+  __syncthreads();
+  if (b == 0) {
+    uint32_t run = 0;
+    for (uint32_t i = 0; i <= nb1; ++i) {
+      const uint32_t c = s[i];
+      s[i] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = b; i <= nb1; i += 256) tile_start[i] = s[i];
+}
+
+DK_DEV uint32_t find_bucket(const uint32_t *ts, uint32_t nb1, uint32_t tile) {
+  uint32_t lo = 0, hi = nb1;  // largest b with ts[b] <= tile, among b < nb1
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ts[mid] <= tile) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_part2(const KeyT *in, KeyT *out, MsdArgs m,
+                                              const uint64_t *bucket_start, const uint64_t *n_dev) {
+  constexpr int TILE = NT * ITEMS;
+  __shared__ KeyT s_keys[TILE];
+  __shared__ uint32_t s_h[2048 + 1];
+  __shared__ uint32_t s_ts[257];
+  __shared__ uint32_t s_wave[NT / 64], s_total;
+  const uint32_t tid = threadIdx.x, nd2 = 1u << m.b2;
+  for (uint32_t i = tid; i <= m.nb1; i += NT) s_ts[i] = m.tile_start[i];
+  for (uint32_t i = tid; i < nd2; i += NT) s_h[i] = 0;
+  __syncthreads();
+  const uint32_t tile = blockIdx.x;
+  if (tile >= s_ts[m.nb1]) return;
+  const uint32_t b = find_bucket(s_ts, m.nb1, tile);
+  const uint64_t bs = bucket_start[b];
+  const uint64_t be = b + 1 < m.nb1 ? bucket_start[b + 1] : n_dev[0];
+  const uint32_t j = tile - s_ts[b], ntb = s_ts[b + 1] - s_ts[b];
+  const uint64_t start = bs + (uint64_t)j * TILE;
+  const uint32_t cnt = (uint32_t)((be - start) < (uint64_t)TILE ? (be - start) : TILE);
+  KeyT keys[ITEMS];
+  uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t idx = tid + i * NT;
+    if (idx < cnt) {
+      keys[i] = in[start + idx];
+      dig[i] = digit_eval(m.d2, keys[i]);
+      rank[i] = atomicAdd(&s_h[dig[i]], 1u);
+    }
+  }
+  __syncthreads();
+  scan_lds<NT>(s_h, nd2, s_wave, &s_total);
+  // segment starts go to this bucket's [digit][tile] block so that one
+  // sub-bucket later reads its starts over all tiles contiguously
+  uint16_t *off = m.offsets + (uint64_t)s_ts[b] * (nd2 + 1) + j;
+  for (uint32_t i = tid; i <= nd2; i += NT) off[(uint64_t)i * ntb] = (uint16_t)s_h[i];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (tid + i * NT < cnt) s_keys[s_h[dig[i]] + rank[i]] = keys[i];
+  __syncthreads();
+  for (uint32_t i = tid; i < cnt; i += NT) out[start + i] = s_keys[i];
+}
+
+// Exclusive prefix of sub-bucket `idx` over all earlier ones, one wavefront
+// reading 64 predecessors' flags at a time.  Called by a whole wavefront.
+DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) {
+  const uint32_t lane = __lane_id();
+  uint64_t excl = 0;
+  int64_t base = (int64_t)idx - 1;
+  while (base >= 0) {
+    const int64_t t = base - (int64_t)lane;
+    uint64_t val = 0;
+    uint32_t st = LB_INC;
+    if (t >= 0) {
+      for (;;) {
+        const uint64_t w = lb_load(flags + t);
+        st = (uint32_t)((w >> 48) & 3);
+        if ((uint32_t)(w >> 50) == epoch && st != 0) {
+          val = w & 0xFFFFFFFFFFFFULL;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    const uint64_t inc = __ballot(st == LB_INC);
+    const uint32_t first = inc ? (uint32_t)__ffsll((unsigned long long)inc) - 1 : 64u;
+    uint64_t v = lane <= first ? val : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    excl += v;
+    if (inc) break;
+    base -= 64;
+  }
+  return excl;
+}
+
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, uint32_t *out_counts,
+                                              MsdArgs m, const uint64_t *bucket_start,
+                                              uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
+                                              uint32_t epoch, uint32_t *err) {
+  constexpr int CAP = NT * ITEMS;  // also the k_part2 tile size
+  __shared__ KeyT B[CAP];
+  __shared__ union {
+    struct {
+      uint32_t pref[NT];
+      uint16_t o0[NT];
+    } seg;
+    uint16_t hp[CAP];
+  } s_u;
+  __shared__ uint32_t s_h[2048 + 1];
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb;
+  __shared__ uint64_t s_excl;
+  uint16_t *s_hp = s_u.hp;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, nd2 = 1u << m.b2;
+  if (tid == 0) s_sb = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t sb = s_sb, nsb = m.nb1 << m.b2;
+  if (sb >= nsb) return;
+  const uint32_t b = sb >> m.b2, s = sb & (nd2 - 1);
+  const uint32_t ts = m.tile_start[b], te = m.tile_start[b + 1], ntb = te - ts;
+  const uint64_t bs = bucket_start[b];
+  const uint16_t *row0 = m.offsets + (uint64_t)ts * (nd2 + 1) + (uint64_t)s * ntb;
+  const uint16_t *row1 = row0 + ntb;
+
+  // gather the sub-bucket's segments from every tile of its bucket: segment
+  // starts are contiguous per digit; keys are copied with independent loads
+  uint32_t mm = 0;
+  for (uint32_t c = 0; c < ntb; c += NT) {
+    const uint32_t nt = ntb - c < (uint32_t)NT ? ntb - c : (uint32_t)NT;
+    uint32_t len = 0, o0 = 0;
+    if (tid < nt) {
+      o0 = row0[c + tid];
+      len = row1[c + tid] - o0;
+    }
+    const uint32_t pre = scan_block<NT>(len, s_wave, &s_total);
+    const uint32_t tot = s_total;
+    if (tid < nt) {
+      s_u.seg.pref[tid] = pre;
+      s_u.seg.o0[tid] = (uint16_t)o0;
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < tot; idx += NT) {
+      uint32_t lo = 0, hi = nt;  // largest q with pref[q] <= idx
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_u.seg.pref[mid] <= idx) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t dst = mm + idx;
+      if (dst < (uint32_t)CAP)
+        B[dst] = in[bs + (uint64_t)(c + lo) * CAP + s_u.seg.o0[lo] + (idx - s_u.seg.pref[lo])];
+    }
+    mm += tot;
+    __syncthreads();
+  }
+  const bool overflow = mm > (uint32_t)CAP;
+  if (overflow && tid == 0) atomicOr(err, ERR_OVERFLOW);
+  const uint32_t mcnt = overflow ? 0 : mm;
+  __syncthreads();
+
+  // LDS partition by the next b3 bits (unstable), then sort each small bin
+  const uint32_t nd3 = 1u << m.d3.nbits;
+  for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
+  KeyT keys[ITEMS];
+  uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t idx = tid + i * NT;
+    if (idx < mcnt) {
+      keys[i] = B[idx];
+      dig[i] = digit_eval(m.d3, keys[i]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (tid + i * NT < mcnt) rank[i] = atomicAdd(&s_h[dig[i]], 1u);
+  __syncthreads();
+  scan_lds<NT>(s_h, nd3, s_wave, &s_total);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (tid + i * NT < mcnt) B[s_h[dig[i]] + rank[i]] = keys[i];
+  __syncthreads();
+  // Order each bin by counting: a key's place is the number of keys of its
+  // bin that are smaller, plus the equal ones stored before it.  All reads are
+  // independent (no chain of dependent moves), and copies of one k-mer keep
+  // their relative order.  A bin longer than RANK_MAX is only accepted when
+  // all its keys are equal (its order is then already right); otherwise the
+  // job is flagged and recounted on the LSD path.
+  constexpr uint32_t RANK_MAX = 128;
+  uint32_t newpos[ITEMS];
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t i = tid + q * NT;
+    newpos[q] = i;
+    if (i < mcnt) {
+      const KeyT x = B[i];
+      keys[q] = x;
+      const uint32_t j = digit_eval(m.d3, x);
+      const uint32_t lo = s_h[j], hi = s_h[j + 1];
+      if (hi - lo <= RANK_MAX) {
+        uint32_t r = 0;
+        for (uint32_t t = lo; t < hi; ++t) {
+          const KeyT y = B[t];
+          r += key_lt(y, x) || (t < i && key_eq(y, x));
+        }
+        newpos[q] = lo + r;
+      } else if (!key_eq(B[lo], x)) {
+        bad = true;  // a long bin of different keys
+      }
+    }
+  }
+  if (bad) atomicOr(err, ERR_OVERFLOW);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q)
+    if (tid + q * NT < mcnt) B[newpos[q]] = keys[q];
+  __syncthreads();
+
+  // run-length reduce inside LDS
+  uint32_t heads = 0;
+  uint32_t hmask = 0;
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t i = tid * ITEMS + q;
+    const bool h = i < mcnt && (i == 0 || !key_eq(B[i], B[i - 1]));
+    hmask |= (h ? 1u : 0u) << q;
+    heads += h;
+  }
+  uint32_t o = scan_block<NT>(heads, s_wave, &s_total);
+  const uint32_t H = s_total;
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q)
+    if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
+  if (wave == 0) {
+    if (tid == 0) lb_store(flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));
+    uint64_t excl = 0;
+    if (sb > 0) {
+      excl = lb_lookback_wave(flags, sb, epoch);
+      if (tid == 0) lb_store(flags + sb, lb_pack(epoch, LB_INC, excl + H));
+    }
+    if (tid == 0) {
+      s_excl = excl;
+      if (sb == nsb - 1) n_dev[1] = excl + H;
+    }
+  }
+  __syncthreads();
+  const uint64_t excl = s_excl;
+  for (uint32_t h = tid; h < H; h += NT) {
+    const uint32_t i = s_hp[h];
+    const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;
+    out_keys[excl + h] = B[i];
+    out_counts[excl + h] = end - i;
+  }
+}
+
 // ------------------------------------------------------------- launchers --
 
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
@@ -501,14 +809,13 @@ void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
   k_fill_u64<<<(uint32_t)g, 256, 0, st>>>(p, v, n);
 }
 
-void launch_hist_reads(int kw, const ReadsDesc &rd, int ndig, int owner_digit, const Workspace &ws,
+void launch_hist_reads(int kw, const ReadsDesc &rd, const HistSpecs &hs, const Workspace &ws,
                        uint32_t ntiles, int grid, hipStream_t st) {
   if (kw == 1)
     k_hist_reads<uint64_t, SWEEP_NT, SWEEP_ITEMS_1>
-        <<<grid, SWEEP_NT, 0, st>>>(rd, ndig, owner_digit, ntiles, ws.hist, ws.err);
+        <<<grid, SWEEP_NT, 0, st>>>(rd, hs, ntiles, ws.hist, ws.err);
   else
-    k_hist_reads<K2, SWEEP_NT, SWEEP_ITEMS_2>
-        <<<grid, SWEEP_NT, 0, st>>>(rd, ndig, owner_digit, ntiles, ws.hist, ws.err);
+    k_hist_reads<K2, SWEEP_NT, SWEEP_ITEMS_2><<<grid, SWEEP_NT, 0, st>>>(rd, hs, ntiles, ws.hist, ws.err);
 }
 
 void launch_scan_digits(int npasses, const Workspace &ws, hipStream_t st) {
@@ -516,9 +823,8 @@ void launch_scan_digits(int npasses, const Workspace &ws, hipStream_t st) {
 }
 
 void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, void *out,
-                  uint32_t shift, bool owner_pass, uint32_t num_pes, const uint64_t *digit_start,
-                  const Workspace &ws, uint32_t ticket_slot, uint32_t epoch, uint32_t grid,
-                  hipStream_t st) {
+                  const DigitSpec &dig, const uint64_t *digit_start, const Workspace &ws,
+                  uint32_t ticket_slot, uint32_t epoch, uint32_t grid, hipStream_t st) {
   SweepArgs a;
   a.rd = rd;
   a.in = in;
@@ -527,9 +833,7 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
   a.digit_start = digit_start;
   a.flags = ws.flags;
   a.ticket = ws.tickets + ticket_slot;
-  a.shift = shift;
-  a.owner_pass = owner_pass ? 1 : 0;
-  a.num_pes = num_pes;
+  a.dig = dig;
   a.epoch = epoch;
   if (!grid) return;
   if (kw == 1) {
@@ -561,6 +865,35 @@ void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const 
                                                ws.bin_offsets);
   else
     k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
+}
+
+void launch_bucket_tiles(int kw, const MsdArgs &m, const Workspace &ws, hipStream_t st) {
+  k_bucket_tiles<<<1, 256, 0, st>>>(ws.hist, m.nb1, (uint32_t)msd_tile(kw), m.tile_start);
+}
+
+void launch_part2(int kw, const void *in, void *out, const MsdArgs &m, const Workspace &ws,
+                  uint32_t grid, hipStream_t st) {
+  if (!grid) return;
+  if (kw == 1)
+    k_part2<uint64_t, MSD_NT, MSD_ITEMS_1><<<grid, MSD_NT, 0, st>>>(
+        (const uint64_t *)in, (uint64_t *)out, m, ws.digit_start, ws.n_dev);
+  else
+    k_part2<K2, MSD_NT, MSD_ITEMS_2><<<grid, MSD_NT, 0, st>>>((const K2 *)in, (K2 *)out, m,
+                                                                    ws.digit_start, ws.n_dev);
+}
+
+void launch_final(int kw, const void *in, void *out_keys, const MsdArgs &m, const Workspace &ws,
+                  uint32_t epoch, hipStream_t st) {
+  const uint32_t nsb = m.nb1 << m.b2;
+  uint32_t *ticket = ws.tickets + MAX_PASSES + 1;
+  if (kw == 1)
+    k_final<uint64_t, MSD_NT, MSD_ITEMS_1><<<nsb, MSD_NT, 0, st>>>(
+        (const uint64_t *)in, (uint64_t *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags,
+        ticket, epoch, ws.err);
+  else
+    k_final<K2, MSD_NT, MSD_ITEMS_2><<<nsb, MSD_NT, 0, st>>>(
+        (const K2 *)in, (K2 *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags, ticket,
+        epoch, ws.err);
 }
 
 }  // namespace dk

@@ -51,7 +51,8 @@ static size_t slots_footprint(const ServeState &s) {
   for (auto &sl : s.slots)
     for (const DevBuf *d : {&sl->ws.hist, &sl->ws.dstart, &sl->ws.ndev, &sl->ws.err, &sl->ws.tickets,
                             &sl->ws.flags, &sl->ws.rle_flags, &sl->ws.keys_a, &sl->ws.keys_b,
-                            &sl->ws.counts, &sl->ws.binoff, &sl->ws.reads})
+                            &sl->ws.counts, &sl->ws.binoff, &sl->ws.reads, &sl->ws.msd_ts,
+                            &sl->ws.msd_off})
       b += d->cap;
   return b;
 }
@@ -61,7 +62,7 @@ static size_t slot_growth(const Slot &sl, const Plan &pl, bool need_reads) {
   const size_t need = job_device_bytes(pl) - (need_reads ? 0 : pl.nbytes);
   size_t have = 0;
   for (const DevBuf *d : {&sl.ws.flags, &sl.ws.rle_flags, &sl.ws.keys_a, &sl.ws.keys_b,
-                          &sl.ws.counts, &sl.ws.reads})
+                          &sl.ws.counts, &sl.ws.reads, &sl.ws.msd_off})
     have += d->cap;
   return need > have ? need - have : 0;
 }
@@ -71,6 +72,20 @@ static void complete(ServeState &s, Slot &sl) {
   Ticket &t = *s.tickets[sl.ticket];
   uint64_t n = 0, d = 0;
   int st = finish_count(sl.ws, t.plan, &n, &d);
+  if (st == STATUS_MSD_OVERFLOW && t.plan.msd) {
+    // a sub-bucket did not fit in LDS: count this job again on the LSD path
+    plan_choose_path(&t.plan, DAKC_ALGO_LSD);
+    const bool need_reads = !t.job.reads_on_device;
+    const uint8_t *dreads = need_reads ? (const uint8_t *)sl.ws.reads.p : (const uint8_t *)t.job.reads;
+    st = sl.ws.reserve(t.plan, need_reads);
+    if (!st && need_reads && t.plan.nbytes &&
+        hipMemcpyAsync(sl.ws.reads.p, t.job.reads, t.plan.nbytes, hipMemcpyHostToDevice,
+                       sl.stream) != hipSuccess)
+      st = DAKC_E_DEVICE;
+    if (!st) st = enqueue_count(sl.ws, t.plan, dreads, s.ctx->num_cus, false, &sl.tkeys);
+    if (!st && hipStreamSynchronize(sl.stream) != hipSuccess) st = DAKC_E_DEVICE;
+    if (!st) st = finish_count(sl.ws, t.plan, &n, &d);
+  }
   if (!st) {
     dakc_table &o = t.table;
     o.n = d;
@@ -129,6 +144,7 @@ static void scheduler(ServeState *s) {
             sl->ws.reads.release();
             sl->ws.flags.release();
             sl->ws.rle_flags.release();
+            sl->ws.msd_off.release();
           }
         if (slots_footprint(*s) + best > s->budget) break;  // wait for running jobs
       }
@@ -244,6 +260,7 @@ extern "C" int dakc_submit(dakc_ctx *ctx, const dakc_job *job, uint64_t *ticket)
   ServeState *s = serve_get(ctx, &st);
   if (!s) return st;
   if (job_device_bytes(pl) > s->budget) return DAKC_E_REJECTED;
+  plan_choose_path(&pl, ctx->opts.algo);
   auto t = std::make_unique<Ticket>();
   t->job = *job;
   t->plan = pl;

@@ -86,10 +86,14 @@ typedef struct dakc_table {
 
 typedef struct dakc_ctx dakc_ctx;
 
+#define DAKC_ALGO_AUTO 0 /* MSD partition + in-LDS count, LSD radix sort when a bucket overflows */
+#define DAKC_ALGO_LSD 1  /* LSD radix sort of all k-mers, then run-length reduce */
+#define DAKC_ALGO_MSD 2  /* same as AUTO (the LSD rerun still guards skewed inputs) */
+
 typedef struct dakc_opts {
   uint64_t hbm_budget;  /* served mode: device bytes the job pool may hold (0 = 60% of free) */
   int32_t max_inflight; /* served mode: concurrent jobs on the device (0 = 4) */
-  int32_t reserved;
+  int32_t algo;         /* DAKC_ALGO_* */
 } dakc_opts;
 
 /* Stage timings of the last dakc_count/dakc_count_device on a context,
@@ -100,7 +104,11 @@ typedef struct dakc_stats {
   uint32_t tiles0;      /* tiles of pass 0 */
   uint64_t n;           /* k-mers */
   uint64_t d;           /* distinct */
-  double sweep_ms[18];  /* per radix pass */
+  double sweep_ms[18];  /* per radix pass (LSD path) */
+  uint32_t path;        /* 1: LSD radix sort, 2: MSD partition + in-LDS count */
+  uint32_t msd_reruns;  /* 1 when an MSD bucket overflowed LDS and the LSD path reran */
+  double part2_ms;      /* MSD: second-level tile partition */
+  double final_ms;      /* MSD: gather + in-LDS sort + run-length reduce */
 } dakc_stats;
 
 const char *dakc_status_str(int status);

@@ -14,9 +14,10 @@ from dakc_amd import DakcError, KmerCounter, _lib
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def kc():
-    c = KmerCounter()
+@pytest.fixture(scope="module", params=[_lib.ALGO_AUTO, _lib.ALGO_LSD], ids=["msd", "lsd"])
+def kc(request):
+    c = KmerCounter(algo=request.param)
+    c.algo = request.param
     yield c
     c.close()
 
@@ -86,6 +87,8 @@ def test_iid_reads_100k(kc):
     data = _lib.generate_reads(p)
     t = run_both(kc, data, 150, 31)
     assert t.total == 100_000 * 120
+    st = kc.last_stats()
+    assert st.path == (2 if kc.algo == _lib.ALGO_AUTO else 1) and st.msd_reruns == 0
 
 
 def test_genome_reads_repeats(kc):
@@ -121,6 +124,11 @@ def test_runs_longer_than_a_tile(kc):
     polya = (("A" * 150 + "\n") * 2000).encode()
     t = run_both(kc, polya, 150, 31)
     assert len(t) == 1 and t.counts[0] == 2000 * 120
+    # one k-mer fills a whole sub-bucket: the MSD path must fall back to LSD
+    st = kc.last_stats()
+    assert st.path == 1
+    if kc.algo == _lib.ALGO_AUTO:
+        assert st.msd_reruns == 1
 
 
 def test_empty_and_short(kc):
@@ -180,6 +188,18 @@ def test_device_resident_input(kc):
         kc._libh.dakc_table_free(ctypes.byref(t))
     finally:
         kc.device_free(dptr)
+
+
+def test_low_complexity_prefixes(kc):
+    # many distinct k-mers sharing long prefixes land in the same in-LDS bin
+    rng = random.Random(5)
+    recs = []
+    for _ in range(3000):
+        tail = "".join(rng.choice("ACGT") for _ in range(40))
+        recs.append("A" * 110 + tail + "\n")
+    data = "".join(recs).encode()
+    run_both(kc, data, 150, 31)
+    run_both(kc, data, 150, 21, canonical=True)
 
 
 def test_repeat_runs_identical(kc):

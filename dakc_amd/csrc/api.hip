@@ -218,14 +218,14 @@ int make_plan(const dakc_job *job, Plan *pl) {
 
 // MSD geometry: first-level bits b1 (<= 8, fewer when owner bins share the
 // 256 first-level buckets), second-level b2 (<= 11), chosen so an average
-// sub-bucket holds at most 60% of what one workgroup sorts in LDS; b3 (<= 11)
+// sub-bucket holds at most 60% of what one workgroup sorts in LDS; b3 (<= 10)
 // splits a sub-bucket inside LDS into bins short enough to order by counting.
 int plan_choose_path(Plan *pl, int algo) {
   Plan &p = *pl;
   p.msd = 0;
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
-  const uint64_t cap = (uint64_t)msd_tile(p.kw);
+  const uint64_t cap = (uint64_t)fin_cap(p.kw), tile2 = (uint64_t)msd_tile(p.kw);
   const uint64_t target = cap * 3 / 5;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
@@ -238,10 +238,10 @@ int plan_choose_path(Plan *pl, int algo) {
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
   p.b1 = b1;
   p.b2 = b2;
-  p.b3 = std::min<uint32_t>(11, bits - b1 - b2);
+  p.b3 = std::min<uint32_t>(10, bits - b1 - b2);
   p.nb1 = owners << b1;
   p.nsb = p.nb1 << b2;
-  p.tiles2 = (uint32_t)((p.nmax + cap - 1) / cap) + p.nb1;
+  p.tiles2 = (uint32_t)((p.nmax + tile2 - 1) / tile2) + p.nb1;
   p.msd = 1;
   return 1;
 }
@@ -293,6 +293,7 @@ static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd, int num_c
   m.b2 = pl.b2;
   m.d2 = {DIG_BITS, bits - pl.b1 - pl.b2, pl.b2, 1};
   m.d3 = {DIG_BITS, bits - pl.b1 - pl.b2 - pl.b3, pl.b3, 1};
+  m.tile2 = (uint32_t)msd_tile(pl.kw);
   m.tile_start = (uint32_t *)ws.msd_ts.p;
   m.offsets = (uint16_t *)ws.msd_off.p;
   launch_bucket_tiles(pl.kw, m, w, st);

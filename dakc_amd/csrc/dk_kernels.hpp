@@ -66,12 +66,17 @@ constexpr int MAX_READ_LEN = 4096;  // keeps L - k + 1 <= keys per tile
 
 inline int tile_keys(int kw) { return SWEEP_NT * (kw == 1 ? SWEEP_ITEMS_1 : SWEEP_ITEMS_2); }
 
-// MSD second level and in-LDS count: one 1024-thread workgroup per CU holds
-// a whole tile / sub-bucket (15360 u64 or 8192 2x u64 keys) in LDS
+// MSD second level: one 1024-thread workgroup per CU splits a tile of 15360
+// u64 (8192 2x u64) keys in LDS.  In-LDS count: a 512-thread workgroup holds a
+// sub-bucket of up to 7680 u64 (4096 2x u64) keys, two workgroups per CU.
 constexpr int MSD_NT = 1024;
 constexpr int MSD_ITEMS_1 = 15;
 constexpr int MSD_ITEMS_2 = 8;
 inline int msd_tile(int kw) { return MSD_NT * (kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2); }
+constexpr int FIN_NT = 512;
+constexpr int FIN_ITEMS_1 = 15;
+constexpr int FIN_ITEMS_2 = 8;
+inline int fin_cap(int kw) { return FIN_NT * (kw == 1 ? FIN_ITEMS_1 : FIN_ITEMS_2); }
 
 // Launchers (all asynchronous on `st`).
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
@@ -94,6 +99,7 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
 struct MsdArgs {
   uint32_t nb1;         // first-level buckets
   uint32_t b2;          // second-level digit bits
+  uint32_t tile2;       // keys per k_part2 tile
   DigitSpec d2, d3;     // second-level digit, in-LDS partition digit
   uint32_t *tile_start; // nb1 + 1 prefix of tiles per bucket
   uint16_t *offsets;    // per bucket b: [2^b2 + 1][tiles of b] exclusive segment starts

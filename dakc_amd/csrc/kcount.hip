@@ -637,7 +637,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
                                               MsdArgs m, const uint64_t *bucket_start,
                                               uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
                                               uint32_t epoch, uint32_t *err) {
-  constexpr int CAP = NT * ITEMS;  // also the k_part2 tile size
+  constexpr int CAP = NT * ITEMS;
   __shared__ KeyT B[CAP];
   __shared__ union {
     struct {
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
     } seg;
     uint16_t hp[CAP];
   } s_u;
-  __shared__ uint32_t s_h[2048 + 1];
+  __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total, s_sb;
   __shared__ uint64_t s_excl;
   uint16_t *s_hp = s_u.hp;
@@ -678,16 +678,41 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
       s_u.seg.o0[tid] = (uint16_t)o0;
     }
     __syncthreads();
-    for (uint32_t idx = tid; idx < tot; idx += NT) {
-      uint32_t lo = 0, hi = nt;  // largest q with pref[q] <= idx
+    // blocked copy: this thread fills positions [p0, p0 + per); one binary
+    // search finds its first segment, later positions walk forward, and every
+    // key load is independent of the others
+    const uint32_t per = (tot + NT - 1) / NT;
+    const uint32_t p0 = tid * per;
+    if (p0 < tot && mm + tot <= (uint32_t)CAP) {  // per <= ITEMS here
+      uint32_t lo = 0, hi = nt;  // largest q with pref[q] <= p0
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_u.seg.pref[mid] <= idx) lo = mid;
+        if (s_u.seg.pref[mid] <= p0) lo = mid;
         else hi = mid;
       }
-      const uint32_t dst = mm + idx;
-      if (dst < (uint32_t)CAP)
-        B[dst] = in[bs + (uint64_t)(c + lo) * CAP + s_u.seg.o0[lo] + (idx - s_u.seg.pref[lo])];
+      uint32_t q = lo;
+      uint32_t qnext = q + 1 < nt ? s_u.seg.pref[q + 1] : tot;
+      const uint32_t pend = p0 + per < tot ? p0 + per : tot;
+      uint64_t src[ITEMS];
+#pragma unroll
+      for (int u = 0; u < ITEMS; ++u) {
+        const uint32_t p = p0 + u;
+        src[u] = 0;
+        if (p < pend) {
+          while (p >= qnext) {  // zero-length segments share their start
+            ++q;
+            qnext = q + 1 < nt ? s_u.seg.pref[q + 1] : tot;
+          }
+          src[u] = bs + (uint64_t)(c + q) * m.tile2 + s_u.seg.o0[q] + (p - s_u.seg.pref[q]);
+        }
+      }
+      KeyT v[ITEMS];
+#pragma unroll
+      for (int u = 0; u < ITEMS; ++u)
+        if (p0 + u < pend) v[u] = in[src[u]];
+#pragma unroll
+      for (int u = 0; u < ITEMS; ++u)
+        if (p0 + u < pend) B[mm + p0 + u] = v[u];
     }
     mm += tot;
     __syncthreads();
@@ -723,21 +748,34 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
   // Order each bin by counting: a key's place is the number of keys of its
   // bin that are smaller, plus the equal ones stored before it.  All reads are
   // independent (no chain of dependent moves), and copies of one k-mer keep
-  // their relative order.  A bin longer than RANK_MAX is only accepted when
-  // all its keys are equal (its order is then already right); otherwise the
-  // job is flagged and recounted on the LSD path.
+  // their relative order.  Bins whose keys are all equal stay as they are; a
+  // mixed bin longer than RANK_MAX flags the job, which is then recounted on
+  // the LSD path.
   constexpr uint32_t RANK_MAX = 128;
+  // bins whose keys are all equal (copies of one k-mer) are already in order
+  __shared__ uint32_t s_mixed[1024 / 32];
+  for (uint32_t w = tid; w < 1024 / 32; w += NT) s_mixed[w] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const uint32_t i = tid + q * NT;
+    if (i < mcnt) {
+      const KeyT x = B[i];
+      keys[q] = x;
+      dig[q] = digit_eval(m.d3, x);
+      if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
+    }
+  }
+  __syncthreads();
   uint32_t newpos[ITEMS];
   bool bad = false;
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
     const uint32_t i = tid + q * NT;
     newpos[q] = i;
-    if (i < mcnt) {
-      const KeyT x = B[i];
-      keys[q] = x;
-      const uint32_t j = digit_eval(m.d3, x);
-      const uint32_t lo = s_h[j], hi = s_h[j + 1];
+    if (i < mcnt && ((s_mixed[dig[q] >> 5] >> (dig[q] & 31)) & 1)) {
+      const KeyT x = keys[q];
+      const uint32_t lo = s_h[dig[q]], hi = s_h[dig[q] + 1];
       if (hi - lo <= RANK_MAX) {
         uint32_t r = 0;
         for (uint32_t t = lo; t < hi; ++t) {
@@ -745,7 +783,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
           r += key_lt(y, x) || (t < i && key_eq(y, x));
         }
         newpos[q] = lo + r;
-      } else if (!key_eq(B[lo], x)) {
+      } else {
         bad = true;  // a long bin of different keys
       }
     }
@@ -887,11 +925,11 @@ void launch_final(int kw, const void *in, void *out_keys, const MsdArgs &m, cons
   const uint32_t nsb = m.nb1 << m.b2;
   uint32_t *ticket = ws.tickets + MAX_PASSES + 1;
   if (kw == 1)
-    k_final<uint64_t, MSD_NT, MSD_ITEMS_1><<<nsb, MSD_NT, 0, st>>>(
+    k_final<uint64_t, FIN_NT, FIN_ITEMS_1><<<nsb, FIN_NT, 0, st>>>(
         (const uint64_t *)in, (uint64_t *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags,
         ticket, epoch, ws.err);
   else
-    k_final<K2, MSD_NT, MSD_ITEMS_2><<<nsb, MSD_NT, 0, st>>>(
+    k_final<K2, FIN_NT, FIN_ITEMS_2><<<nsb, FIN_NT, 0, st>>>(
         (const K2 *)in, (K2 *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags, ticket,
         epoch, ws.err);
 }

@@ -80,7 +80,7 @@ void DevBuf::release() {
 
 JobWs::~JobWs() {
   for (auto *b : {&hist, &dstart, &ndev, &err, &tickets, &flags, &rle_flags, &keys_a, &keys_b,
-                  &counts, &binoff, &reads, &msd_ts, &msd_off})
+                  &counts, &binoff, &reads, &msd_off1, &msd_off2})
     b->release();
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
@@ -125,7 +125,13 @@ size_t job_device_bytes(const Plan &pl) {
   b += (size_t)std::max<uint64_t>(pl.tiles0, pl.tiles_keys) * 256 * 8 + (size_t)pl.tiles_keys * 8;
   b += 2 * (size_t)pl.nmax * kb + (size_t)pl.nmax * 4;
   b += (size_t)pl.nbytes + 256;
-  if (pl.msd) b += (size_t)pl.tiles2 * (((size_t)1 << pl.b2) + 1) * 2 + 4096 + (size_t)pl.nsb * 8;
+  if (pl.msd) {
+    b += ((size_t)pl.nb1 + 1) * pl.ntiles1 * 2 + (size_t)pl.nb1 * (pl.nd2 + 1) * pl.ngroups * 2;
+    b += (size_t)pl.nsb * 8;
+    const size_t out1 = (size_t)pl.ntiles1 * pl.tcap1, out2 = (size_t)pl.nb1 * pl.ngroups * pl.cap2;
+    if (out1 > pl.nmax) b += (out1 - pl.nmax) * kb;
+    if (out2 > pl.nmax) b += (out2 - pl.nmax) * kb;
+  }
   return b;
 }
 
@@ -148,15 +154,19 @@ int JobWs::reserve(const Plan &pl, bool need_reads) {
     launch_fill_u64((uint64_t *)rle_flags.p, 0, rle_flags.cap / 8, stream);
     epoch = 0;
   }
-  if ((s = keys_a.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
-  if ((s = keys_b.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
+  size_t na = pl.nmax, nbk = pl.nmax;
+  if (pl.msd) {
+    na = std::max<size_t>(na, (size_t)pl.ntiles1 * pl.tcap1);
+    nbk = std::max<size_t>(nbk, (size_t)pl.nb1 * pl.ngroups * pl.cap2);
+  }
+  if ((s = keys_a.ensure(std::max<size_t>(na, 1) * kb + 256, stream, false))) return s;
+  if ((s = keys_b.ensure(std::max<size_t>(nbk, 1) * kb + 256, stream, false))) return s;
   if ((s = counts.ensure(std::max<size_t>(pl.nmax, 1) * 4 + 256, stream, false))) return s;
   if ((s = binoff.ensure(((size_t)pl.num_pes + 1) * 8 + 64, stream, false))) return s;
   if (need_reads && (s = reads.ensure(pl.nbytes + 256, stream, false))) return s;
   if (pl.msd) {
-    if ((s = msd_ts.ensure(260 * 4, stream, false))) return s;
-    if ((s = msd_off.ensure(std::max<size_t>(pl.tiles2, 1) * (((size_t)1 << pl.b2) + 1) * 2 + 256,
-                            stream, false)))
+    if ((s = msd_off1.ensure(((size_t)pl.nb1 + 1) * pl.ntiles1 * 2 + 256, stream, false))) return s;
+    if ((s = msd_off2.ensure((size_t)pl.nb1 * (pl.nd2 + 1) * pl.ngroups * 2 + 256, stream, false)))
       return s;
   }
   return DAKC_OK;
@@ -216,32 +226,55 @@ int make_plan(const dakc_job *job, Plan *pl) {
   return DAKC_OK;
 }
 
-// MSD geometry: first-level bits b1 (<= 8, fewer when owner bins share the
-// 256 first-level buckets), second-level b2 (<= 11), chosen so an average
-// sub-bucket holds at most 60% of what one workgroup sorts in LDS; b3 (<= 10)
-// splits a sub-bucket inside LDS into bins short enough to order by counting.
+// MSD geometry.  A final sub-bucket should hold at most 60% of what one
+// final workgroup sorts in LDS, which fixes b1 + b2; b1 <= 10 (1024 first-level
+// buckets, fewer key bits when owner bins share them) and b2 <= 10.  L1 tiles
+// take as many records as fill 15360 (8192) slots; an L2 group gathers G
+// consecutive L1 tiles of a bucket, G sized so a group fills 75% of an L2
+// workgroup on average.  b3 (<= 9) splits a sub-bucket into LDS bins.
 int plan_choose_path(Plan *pl, int algo) {
   Plan &p = *pl;
   p.msd = 0;
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
-  const uint64_t cap = (uint64_t)fin_cap(p.kw), tile2 = (uint64_t)msd_tile(p.kw);
-  const uint64_t target = cap * 3 / 5;
+  const uint64_t cap3 = (uint64_t)fin_cap(p.kw), cap2 = (uint64_t)msd_tile(p.kw);
+  const uint64_t target = cap3 * 3 / 5;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
-  uint32_t owners = p.owner_digit ? p.num_pes : 1;
-  if (owners > 256) return 0;
+  const uint32_t owners = p.owner_digit ? p.num_pes : 1;
+  if (owners > 1024) return 0;
   uint32_t b1max = 0;
-  while ((owners << (b1max + 1)) <= 256 && b1max < 8) ++b1max;
-  const uint32_t b1 = std::min(std::min(b1max, b12), bits);
-  const uint32_t b2 = std::min(std::min<uint32_t>(11, b12 - b1), bits - b1);
+  while ((owners << (b1max + 1)) <= 1024 && b1max < 10) ++b1max;
+  uint32_t b1 = std::min(std::min(b1max, (b12 + 1) / 2), bits);
+  if (b12 > b1 + 10) b1 = std::min(b1max, b12 - 10);
+  const uint32_t b2 = std::min(std::min<uint32_t>(10, b12 > b1 ? b12 - b1 : 0), bits - b1);
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
+  // L1 tile geometry
+  const uint32_t wpr = (p.L + 31) / 32 + 2;
+  uint32_t rt = p.S ? (uint32_t)(cap2 / p.S) : (uint32_t)cap2;
+  rt = std::min<uint32_t>(rt, L1_MAXRT);
+  rt = std::min<uint32_t>(rt, L1_RAW / p.stride);
+  rt = std::min<uint32_t>(rt, L1_CODEW / wpr);
+  rt = std::max<uint32_t>(rt, 1);
+  if ((uint64_t)rt * p.S > cap2) return 0;
   p.b1 = b1;
   p.b2 = b2;
-  p.b3 = std::min<uint32_t>(10, bits - b1 - b2);
+  p.b3 = std::min<uint32_t>(9, bits - b1 - b2);
   p.nb1 = owners << b1;
-  p.nsb = p.nb1 << b2;
-  p.tiles2 = (uint32_t)((p.nmax + tile2 - 1) / tile2) + p.nb1;
+  p.nd2 = 1u << b2;
+  p.nsb = p.nb1 * p.nd2;
+  p.RT1 = rt;
+  p.wpr1 = wpr;
+  p.ntiles1 = (uint32_t)((p.nrec + rt - 1) / rt);
+  p.tcap1 = rt * p.S;
+  const double per_tile_bucket = (double)p.nmax / p.ntiles1 / p.nb1;
+  uint32_t G = (uint32_t)std::max(1.0, std::floor(0.75 * (double)cap2 / std::max(per_tile_bucket, 1e-9)));
+  G = std::min<uint32_t>(G, MSD_NT);
+  G = std::min<uint32_t>(G, p.ntiles1);
+  p.G = std::max<uint32_t>(G, 1);
+  p.ngroups = (p.ntiles1 + p.G - 1) / p.G;
+  if (p.ngroups > (uint32_t)FIN_MAX_GROUPS) return 0;
+  p.cap2 = (uint32_t)cap2;
   p.msd = 1;
   return 1;
 }
@@ -267,39 +300,37 @@ ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
 
 // Enqueue the whole count of one job on ws.stream.  The sorted table ends in
 // *table_keys (one of the two key buffers) and ws.counts.
-static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd, int num_cus, bool events,
+static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd0, int num_cus, bool events,
                        void **table_keys) {
+  (void)num_cus;
   hipStream_t st = ws.stream;
   Workspace w = ws.view();
+  ReadsDesc rd = rd0;  // L1 tiles hold RT1 records
+  rd.RT = pl.RT1;
+  rd.wpr = pl.wpr1;
   const uint32_t bits = 2 * (uint32_t)pl.k;
-  DigitSpec d1 = {DIG_MSD, bits - pl.b1, pl.b1, pl.owner_digit ? pl.num_pes : 1u};
-  HistSpecs hs;
-  hs.n = 1;
-  hs.d[0] = d1;
-  launch_fill_u64(w.hist, 0, 256, st);
+  MsdArgs m;
+  m.d1 = {DIG_MSD, bits - pl.b1, pl.b1, pl.owner_digit ? pl.num_pes : 1u};
+  m.d2 = {DIG_BITS, bits - pl.b1 - pl.b2, pl.b2, 1};
+  m.d3 = {DIG_BITS, bits - pl.b1 - pl.b2 - pl.b3, pl.b3, 1};
+  m.nb1 = pl.nb1;
+  m.nd2 = pl.nd2;
+  m.ntiles1 = pl.ntiles1;
+  m.tcap1 = pl.tcap1;
+  m.G = pl.G;
+  m.ngroups = pl.ngroups;
+  m.cap2 = pl.cap2;
+  m.off1 = (uint16_t *)ws.msd_off1.p;
+  m.off2 = (uint16_t *)ws.msd_off2.p;
   launch_fill_u64(w.n_dev, 0, 2, st);
   launch_fill_u32(w.err, 0, 1, st);
   launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
   if (events) hipEventRecord(ws.ev[0], st);
-  const int grid = (int)std::min<uint64_t>(pl.tiles0, (uint64_t)num_cus * 4);
-  launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
-  launch_scan_digits(1, w, st);
+  launch_l1(pl.kw, rd, m, w.keys_a, w, st);
   if (events) hipEventRecord(ws.ev[1], st);
-  launch_sweep(pl.kw, true, rd, nullptr, w.keys_a, d1, w.digit_start, w, 0, ws.next_epoch(),
-               pl.tiles0, st);
+  launch_l2(pl.kw, w.keys_a, w.keys_b, m, w, st);
   if (events) hipEventRecord(ws.ev[2], st);
-  MsdArgs m;
-  m.nb1 = pl.nb1;
-  m.b2 = pl.b2;
-  m.d2 = {DIG_BITS, bits - pl.b1 - pl.b2, pl.b2, 1};
-  m.d3 = {DIG_BITS, bits - pl.b1 - pl.b2 - pl.b3, pl.b3, 1};
-  m.tile2 = (uint32_t)msd_tile(pl.kw);
-  m.tile_start = (uint32_t *)ws.msd_ts.p;
-  m.offsets = (uint16_t *)ws.msd_off.p;
-  launch_bucket_tiles(pl.kw, m, w, st);
-  launch_part2(pl.kw, w.keys_a, w.keys_b, m, w, pl.tiles2, st);
-  if (events) hipEventRecord(ws.ev[3], st);
-  launch_final(pl.kw, w.keys_b, w.keys_a, m, w, ws.next_epoch(), st);
+  launch_fin(pl.kw, w.keys_b, w.keys_a, m, w, ws.next_epoch(), st);
   if (pl.owner_digit) launch_bin_offsets(pl.kw, w.keys_a, pl.num_pes, w, st);
   if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
   *table_keys = w.keys_a;
@@ -429,11 +460,12 @@ static void fill_stats(dakc_stats &st, JobWs &ws, const Plan &pl, uint64_t n, ui
   hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]);
   st.hist_ms = ms;
   if (pl.nmax && pl.msd) {
-    hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);
+    st.hist_ms = 0;
+    hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]);
     st.pass0_ms = st.sweep_ms[0] = ms;
-    hipEventElapsedTime(&ms, ws.ev[2], ws.ev[3]);
+    hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);
     st.part2_ms = ms;
-    hipEventElapsedTime(&ms, ws.ev[3], ws.ev[2 + MAX_PASSES]);
+    hipEventElapsedTime(&ms, ws.ev[2], ws.ev[2 + MAX_PASSES]);
     st.final_ms = ms;
   } else if (pl.nmax) {
     hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]);

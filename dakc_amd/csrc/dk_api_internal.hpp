@@ -34,9 +34,11 @@ struct Plan {
   int32_t owner_bin = -1;
   int ndig = 0, owner_digit = 0, npasses = 0;
   uint32_t canonical = 0, last_has_nl = 1;
-  // MSD path (msd = 1): first-level bits b1, second-level b2, in-LDS b3
+  // MSD path (msd = 1): first-level bits b1 (with the owner when all bins
+  // are counted), second-level b2, in-LDS b3; L1 tiles of RT1 records
   int msd = 0;
-  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nsb = 1, tiles2 = 0;
+  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1;
+  uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
@@ -48,7 +50,7 @@ int plan_choose_path(Plan *pl, int algo);
 // Device workspace of one in-flight job (one per served slot).
 struct JobWs {
   DevBuf hist, dstart, ndev, err, tickets, flags, rle_flags, keys_a, keys_b, counts, binoff, reads,
-      msd_ts, msd_off;
+      msd_off1, msd_off2;
   hipStream_t stream = nullptr;
   hipEvent_t ev[MAX_PASSES + 4] = {};
   uint32_t epoch = 0;

@@ -66,16 +66,20 @@ constexpr int MAX_READ_LEN = 4096;  // keeps L - k + 1 <= keys per tile
 
 inline int tile_keys(int kw) { return SWEEP_NT * (kw == 1 ? SWEEP_ITEMS_1 : SWEEP_ITEMS_2); }
 
-// MSD second level: one 1024-thread workgroup per CU splits a tile of 15360
-// u64 (8192 2x u64) keys in LDS.  In-LDS count: a 512-thread workgroup holds a
-// sub-bucket of up to 7680 u64 (4096 2x u64) keys, two workgroups per CU.
+// MSD levels 1 and 2: one 1024-thread workgroup per CU handles a tile of
+// 15360 u64 (8192 2x u64) keys in LDS.  Final level: 256-thread workgroups,
+// four per CU, each holding a sub-bucket of up to 3840 u64 (2048 2x u64) keys.
 constexpr int MSD_NT = 1024;
 constexpr int MSD_ITEMS_1 = 15;
 constexpr int MSD_ITEMS_2 = 8;
 inline int msd_tile(int kw) { return MSD_NT * (kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2); }
-constexpr int FIN_NT = 512;
+constexpr int L1_RAW = 32768;    // LDS bytes of raw records per L1 tile
+constexpr int L1_CODEW = 1792;   // packed code words per L1 tile
+constexpr int L1_MAXRT = 256;    // records per L1 tile
+constexpr int FIN_NT = 256;
 constexpr int FIN_ITEMS_1 = 15;
 constexpr int FIN_ITEMS_2 = 8;
+constexpr int FIN_MAX_GROUPS = 1024;  // L2 groups per bucket the final level can gather
 inline int fin_cap(int kw) { return FIN_NT * (kw == 1 ? FIN_ITEMS_1 : FIN_ITEMS_2); }
 
 // Launchers (all asynchronous on `st`).
@@ -93,22 +97,31 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
                   const DigitSpec &dig, const uint64_t *digit_start, const Workspace &ws,
                   uint32_t ticket_slot, uint32_t epoch, uint32_t grid, hipStream_t st);
 
-// MSD path: after an MSD sweep from the records into nb1 buckets, split every
-// bucket's tiles locally by a second digit, then count each (bucket, digit)
-// sub-bucket inside LDS.
+// MSD path (three tile-local levels, no global histogram):
+//   k_l1  records -> k-mers in first-level bucket order, per tile, with the
+//         tile's bucket starts (off1[bucket][tile])
+//   k_l2  bucket b, group of G consecutive L1 tiles -> keys in second-level
+//         digit order, with the digit starts (off2[b][digit][group])
+//   k_fin sub-bucket (b, digit) gathered from b's groups into LDS, sorted,
+//         run-length reduced, written in place by the decoupled look-back
 struct MsdArgs {
-  uint32_t nb1;         // first-level buckets
-  uint32_t b2;          // second-level digit bits
-  uint32_t tile2;       // keys per k_part2 tile
-  DigitSpec d2, d3;     // second-level digit, in-LDS partition digit
-  uint32_t *tile_start; // nb1 + 1 prefix of tiles per bucket
-  uint16_t *offsets;    // per bucket b: [2^b2 + 1][tiles of b] exclusive segment starts
+  DigitSpec d1, d2, d3;
+  uint32_t nb1;      // first-level buckets (owner x 2^b1)
+  uint32_t nd2;      // second-level digits (2^b2)
+  uint32_t ntiles1;  // L1 tiles
+  uint32_t tcap1;    // keys capacity of one L1 tile (records per tile * slots)
+  uint32_t G;        // L1 tiles per L2 group
+  uint32_t ngroups;  // L2 groups per bucket
+  uint32_t cap2;     // keys capacity of one (bucket, group)
+  uint16_t *off1;    // [nb1 + 1][ntiles1]
+  uint16_t *off2;    // [nb1][nd2 + 1][ngroups]
 };
-void launch_bucket_tiles(int kw, const MsdArgs &m, const Workspace &ws, hipStream_t st);
-void launch_part2(int kw, const void *in, void *out, const MsdArgs &m, const Workspace &ws,
-                  uint32_t grid, hipStream_t st);
-void launch_final(int kw, const void *in, void *out_keys, const MsdArgs &m, const Workspace &ws,
-                  uint32_t epoch, hipStream_t st);
+void launch_l1(int kw, const ReadsDesc &rd, const MsdArgs &m, void *out1, const Workspace &ws,
+               hipStream_t st);
+void launch_l2(int kw, const void *in1, void *out2, const MsdArgs &m, const Workspace &ws,
+               hipStream_t st);
+void launch_fin(int kw, const void *in2, void *out_keys, const MsdArgs &m, const Workspace &ws,
+                uint32_t epoch, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,

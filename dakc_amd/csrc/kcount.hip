@@ -71,23 +71,26 @@ DK_DEV uint32_t scan_block(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
 
 // ------------------------------------------------------------ extraction --
 
-constexpr int MAX_RT = 128;          // records per tile
-constexpr int MAX_CODE_WORDS = 896;  // >= MAX_RT * wpr for every allowed L
-
-struct ExtractSmem {
-  uint32_t raw[RAW_CAP / 4 + 4];
-  uint64_t code[MAX_CODE_WORDS];  // MSB-first 2-bit codes, 32 bases per word
-  uint32_t inv[MAX_CODE_WORDS];   // LSB-first: bit q set = base 32w+q cannot be in a k-mer
-  uint32_t mpos[MAX_RT];          // first 'M' (end of read) per record, L if none
+// LDS scratch of one tile of records: raw bytes, packed 2-bit codes, masks of
+// positions that cannot be in a k-mer, and each record's first 'M'.
+template <int RAWB, int CODEW, int MAXRT>
+struct ExtractSmemT {
+  static constexpr int RAW_BYTES = RAWB, CODE_WORDS = CODEW, MAX_RECS = MAXRT;
+  uint32_t raw[RAWB / 4 + 4];
+  uint64_t code[CODEW];  // MSB-first 2-bit codes, 32 bases per word
+  uint32_t inv[CODEW];   // LSB-first: bit q set = base 32w+q cannot be in a k-mer
+  uint32_t mpos[MAXRT];  // first 'M' (end of read) per record, L if none
 };
+using ExtractSmem = ExtractSmemT<RAW_CAP, 896, 128>;        // 512-thread tiles
+using ExtractSmemBig = ExtractSmemT<L1_RAW, L1_CODEW, L1_MAXRT>;  // 1024-thread tiles
 
 // Decode one tile of records and emit the k-mers of ITEMS slots per thread.
 // Slot s of the tile is (record s / S, start position s % S); thread t holds
 // slots wave*ITEMS*64 + j*64 + lane.  Restates kcounter_funcs.cpp:40-111: a
 // window is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it
 // starts before the record's first 'M' (0xF0 ends the read).
-template <typename KeyT, int NT, int ITEMS, bool CHECK>
-DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, ExtractSmem &sm, KeyT (&keys)[ITEMS],
+template <typename KeyT, int NT, int ITEMS, bool CHECK, typename SM>
+DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[ITEMS],
                          uint32_t &vmask, uint32_t *err) {
   const uint32_t tid = threadIdx.x;
   const uint64_t r0 = (uint64_t)tile * rd.RT;
@@ -525,78 +528,171 @@ DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_tota
   __syncthreads();
 }
 
-// tiles per first-level bucket -> prefix
-__global__ __launch_bounds__(256) void k_bucket_tiles(const uint64_t *hist, uint32_t nb1,
-                                                      uint32_t tile2, uint32_t *tile_start) {
-  __shared__ uint32_t s[257];
-  const uint32_t b = threadIdx.x;
-  s[b] = b < nb1 ? (uint32_t)((hist[b] + tile2 - 1) / tile2) : 0;
-  if (b == 0) s[256] = 0;
-  __syncthreads();
-  if (b == 0) {
-    uint32_t run = 0;
-    for (uint32_t i = 0; i <= nb1; ++i) {
-      const uint32_t c = s[i];
-      s[i] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = b; i <= nb1; i += 256) tile_start[i] = s[i];
+
+// Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range of
+// tiles so the offset words neighbouring tiles write meet in one L2.
+DK_DEV uint32_t xcd_remap(uint32_t bid, uint32_t n) {
+  const uint32_t x = bid & 7, k = bid >> 3, per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + k;
 }
 
-DK_DEV uint32_t find_bucket(const uint32_t *ts, uint32_t nb1, uint32_t tile) {
-  uint32_t lo = 0, hi = nb1;  // largest b with ts[b] <= tile, among b < nb1
+// Gather nseg segments into the workgroup's registers.  Segment q holds
+// row1[q] - row0[q] keys at src[region0 + q * stride + row0[q]].  Thread t
+// receives positions [t*per, t*per + mine) of the concatenation, per =
+// ceil(total / NT) <= ITEMS; one binary search per thread finds its first
+// segment and every key load is independent.  Returns the total, or more than
+// cap (and no keys) when the segments do not fit.
+template <typename KeyT, int NT, int ITEMS, int TPT>
+DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uint16_t *row1,
+                                uint32_t nseg, uint64_t region0, uint64_t stride, uint32_t *s_pref,
+                                uint16_t *s_o0, uint32_t *s_wave, uint32_t *s_total,
+                                KeyT (&v)[ITEMS], uint32_t &mine) {
+  constexpr uint32_t CAP = NT * ITEMS;
+  const uint32_t tid = threadIdx.x;
+  uint32_t o0v[TPT], lenv[TPT], sum = 0;
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const uint32_t q = tid * TPT + u;
+    o0v[u] = lenv[u] = 0;
+    if (q < nseg) {
+      o0v[u] = row0[q];
+      lenv[u] = row1[q] - o0v[u];
+    }
+    sum += lenv[u];
+  }
+  uint32_t pre = scan_block<NT>(sum, s_wave, s_total);
+  const uint32_t tot = *s_total;
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const uint32_t q = tid * TPT + u;
+    if (q < nseg) {
+      s_pref[q] = pre;
+      s_o0[q] = (uint16_t)o0v[u];
+    }
+    pre += lenv[u];
+  }
+  __syncthreads();
+  mine = 0;
+  if (tot > CAP) return tot;
+  const uint32_t per = (tot + NT - 1) / NT;
+  const uint32_t p0 = tid * per;
+  if (p0 >= tot) return tot;
+  mine = per < tot - p0 ? per : tot - p0;
+  uint32_t lo = 0, hi = nseg;  // largest q with pref[q] <= p0
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (ts[mid] <= tile) lo = mid;
+    if (s_pref[mid] <= p0) lo = mid;
     else hi = mid;
   }
-  return lo;
+  uint32_t q = lo;
+  uint32_t qnext = q + 1 < nseg ? s_pref[q + 1] : tot;
+  uint64_t addr[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    const uint32_t p = p0 + u;
+    addr[u] = 0;
+    if ((uint32_t)u < mine) {
+      while (p >= qnext) {  // zero-length segments share their start
+        ++q;
+        qnext = q + 1 < nseg ? s_pref[q + 1] : tot;
+      }
+      addr[u] = region0 + (uint64_t)q * stride + s_o0[q] + (p - s_pref[q]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if ((uint32_t)u < mine) v[u] = src[addr[u]];
+  return tot;
 }
 
+// L1: one tile of records -> its k-mers in first-level bucket order, written
+// contiguously at tile * tcap1, with the tile's bucket starts in off1.
+// Also the record-format checks and the k-mer total.
 template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_part2(const KeyT *in, KeyT *out, MsdArgs m,
-                                              const uint64_t *bucket_start, const uint64_t *n_dev) {
+__global__ __launch_bounds__(NT) void k_l1(ReadsDesc rd, MsdArgs m, KeyT *out1, uint64_t *n_dev,
+                                           uint32_t *err) {
   constexpr int TILE = NT * ITEMS;
-  __shared__ KeyT s_keys[TILE];
-  __shared__ uint32_t s_h[2048 + 1];
-  __shared__ uint32_t s_ts[257];
+  union Smem {
+    ExtractSmemBig ex;
+    KeyT stage[TILE];
+  };
+  __shared__ Smem sm;
+  __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
-  const uint32_t tid = threadIdx.x, nd2 = 1u << m.b2;
-  for (uint32_t i = tid; i <= m.nb1; i += NT) s_ts[i] = m.tile_start[i];
-  for (uint32_t i = tid; i < nd2; i += NT) s_h[i] = 0;
-  __syncthreads();
-  const uint32_t tile = blockIdx.x;
-  if (tile >= s_ts[m.nb1]) return;
-  const uint32_t b = find_bucket(s_ts, m.nb1, tile);
-  const uint64_t bs = bucket_start[b];
-  const uint64_t be = b + 1 < m.nb1 ? bucket_start[b + 1] : n_dev[0];
-  const uint32_t j = tile - s_ts[b], ntb = s_ts[b + 1] - s_ts[b];
-  const uint64_t start = bs + (uint64_t)j * TILE;
-  const uint32_t cnt = (uint32_t)((be - start) < (uint64_t)TILE ? (be - start) : TILE);
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
   KeyT keys[ITEMS];
-  uint32_t dig[ITEMS], rank[ITEMS];
+  uint32_t vmask, dig[ITEMS], rank[ITEMS];
+  extract_tile<KeyT, NT, ITEMS, true>(rd, tile, sm.ex, keys, vmask, err);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t idx = tid + i * NT;
-    if (idx < cnt) {
-      keys[i] = in[start + idx];
-      dig[i] = digit_eval(m.d2, keys[i]);
-      rank[i] = atomicAdd(&s_h[dig[i]], 1u);
+  for (int j = 0; j < ITEMS; ++j) {
+    if ((vmask >> j) & 1) {
+      dig[j] = digit_eval(m.d1, keys[j]);
+      rank[j] = atomicAdd(&s_h[dig[j]], 1u);
     }
   }
   __syncthreads();
-  scan_lds<NT>(s_h, nd2, s_wave, &s_total);
-  // segment starts go to this bucket's [digit][tile] block so that one
-  // sub-bucket later reads its starts over all tiles contiguously
-  uint16_t *off = m.offsets + (uint64_t)s_ts[b] * (nd2 + 1) + j;
-  for (uint32_t i = tid; i <= nd2; i += NT) off[(uint64_t)i * ntb] = (uint16_t)s_h[i];
+  scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
+  const uint32_t total = s_h[m.nb1];
+  uint16_t *off = m.off1 + tile;
+  for (uint32_t d = tid; d <= m.nb1; d += NT) off[(uint64_t)d * m.ntiles1] = (uint16_t)s_h[d];
+  if (tid == 0 && total) atomicAdd(reinterpret_cast<unsigned long long *>(n_dev), (unsigned long long)total);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if (tid + i * NT < cnt) s_keys[s_h[dig[i]] + rank[i]] = keys[i];
+  for (int j = 0; j < ITEMS; ++j)
+    if ((vmask >> j) & 1) sm.stage[s_h[dig[j]] + rank[j]] = keys[j];
   __syncthreads();
-  for (uint32_t i = tid; i < cnt; i += NT) out[start + i] = s_keys[i];
+  KeyT *o = out1 + (uint64_t)tile * m.tcap1;
+  for (uint32_t i = tid; i < total; i += NT) o[i] = sm.stage[i];
+  (void)lane;
+  (void)wave;
+}
+
+// L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
+// digit order, written at (b * ngroups + g) * cap2, with the digit starts in
+// off2[b][digit][g].
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_l2(const KeyT *in1, KeyT *out2, MsdArgs m, uint32_t *err) {
+  constexpr int CAP = NT * ITEMS;
+  __shared__ KeyT stage[CAP];
+  __shared__ uint32_t s_pref[NT];
+  __shared__ uint16_t s_o0[NT];
+  __shared__ uint32_t s_h[1024 + 1];
+  __shared__ uint32_t s_wave[NT / 64], s_total;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t b = wg / m.ngroups, g = wg - b * m.ngroups;
+  const uint32_t t0 = g * m.G;
+  const uint32_t nt = m.ntiles1 - t0 < m.G ? m.ntiles1 - t0 : m.G;
+  const uint16_t *row0 = m.off1 + (uint64_t)b * m.ntiles1 + t0;
+  for (uint32_t i = tid; i <= m.nd2; i += NT) s_h[i] = 0;
+  KeyT v[ITEMS];
+  uint32_t mine;
+  uint32_t tot = gather_segments<KeyT, NT, ITEMS, 1>(in1, row0, row0 + m.ntiles1, nt,
+                                                     (uint64_t)t0 * m.tcap1, m.tcap1, s_pref, s_o0,
+                                                     s_wave, &s_total, v, mine);
+  if (tot > (uint32_t)CAP) {
+    if (tid == 0) atomicOr(err, ERR_OVERFLOW);
+    tot = 0;
+  }
+  uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      dig[u] = digit_eval(m.d2, v[u]);
+      rank[u] = atomicAdd(&s_h[dig[u]], 1u);
+    }
+  }
+  __syncthreads();
+  scan_lds<NT>(s_h, m.nd2, s_wave, &s_total);
+  uint16_t *off = m.off2 + (uint64_t)b * (m.nd2 + 1) * m.ngroups + g;
+  for (uint32_t d = tid; d <= m.nd2; d += NT) off[(uint64_t)d * m.ngroups] = (uint16_t)(tot ? s_h[d] : 0);
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if ((uint32_t)u < mine) stage[s_h[dig[u]] + rank[u]] = v[u];
+  __syncthreads();
+  KeyT *o = out2 + ((uint64_t)b * m.ngroups + g) * m.cap2;
+  for (uint32_t i = tid; i < tot; i += NT) o[i] = stage[i];
 }
 
 // Exclusive prefix of sub-bucket `idx` over all earlier ones, one wavefront
@@ -632,130 +728,68 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
   return excl;
 }
 
+// Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
+// into LDS, split by the next b3 bits, each bin ordered by counting, run-length
+// reduced, and written at the place the decoupled look-back gives.
 template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, uint32_t *out_counts,
-                                              MsdArgs m, const uint64_t *bucket_start,
-                                              uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
-                                              uint32_t epoch, uint32_t *err) {
+__global__ __launch_bounds__(NT) void k_fin(const KeyT *in2, KeyT *out_keys, uint32_t *out_counts,
+                                            MsdArgs m, uint64_t *n_dev, uint64_t *flags,
+                                            uint32_t *ticket, uint32_t epoch, uint32_t *err) {
   constexpr int CAP = NT * ITEMS;
+  constexpr int TPT = FIN_MAX_GROUPS / NT;
   __shared__ KeyT B[CAP];
   __shared__ union {
     struct {
-      uint32_t pref[NT];
-      uint16_t o0[NT];
+      uint32_t pref[FIN_MAX_GROUPS];
+      uint16_t o0[FIN_MAX_GROUPS];
     } seg;
     uint16_t hp[CAP];
   } s_u;
-  __shared__ uint32_t s_h[1024 + 1];
+  __shared__ uint32_t s_h[512 + 1];
+  __shared__ uint32_t s_mixed[512 / 32];
   __shared__ uint32_t s_wave[NT / 64], s_total, s_sb;
   __shared__ uint64_t s_excl;
-  uint16_t *s_hp = s_u.hp;
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, nd2 = 1u << m.b2;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
   if (tid == 0) s_sb = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t sb = s_sb, nsb = m.nb1 << m.b2;
-  if (sb >= nsb) return;
-  const uint32_t b = sb >> m.b2, s = sb & (nd2 - 1);
-  const uint32_t ts = m.tile_start[b], te = m.tile_start[b + 1], ntb = te - ts;
-  const uint64_t bs = bucket_start[b];
-  const uint16_t *row0 = m.offsets + (uint64_t)ts * (nd2 + 1) + (uint64_t)s * ntb;
-  const uint16_t *row1 = row0 + ntb;
-
-  // gather the sub-bucket's segments from every tile of its bucket: segment
-  // starts are contiguous per digit; keys are copied with independent loads
-  uint32_t mm = 0;
-  for (uint32_t c = 0; c < ntb; c += NT) {
-    const uint32_t nt = ntb - c < (uint32_t)NT ? ntb - c : (uint32_t)NT;
-    uint32_t len = 0, o0 = 0;
-    if (tid < nt) {
-      o0 = row0[c + tid];
-      len = row1[c + tid] - o0;
-    }
-    const uint32_t pre = scan_block<NT>(len, s_wave, &s_total);
-    const uint32_t tot = s_total;
-    if (tid < nt) {
-      s_u.seg.pref[tid] = pre;
-      s_u.seg.o0[tid] = (uint16_t)o0;
-    }
-    __syncthreads();
-    // blocked copy: this thread fills positions [p0, p0 + per); one binary
-    // search finds its first segment, later positions walk forward, and every
-    // key load is independent of the others
-    const uint32_t per = (tot + NT - 1) / NT;
-    const uint32_t p0 = tid * per;
-    if (p0 < tot && mm + tot <= (uint32_t)CAP) {  // per <= ITEMS here
-      uint32_t lo = 0, hi = nt;  // largest q with pref[q] <= p0
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_u.seg.pref[mid] <= p0) lo = mid;
-        else hi = mid;
-      }
-      uint32_t q = lo;
-      uint32_t qnext = q + 1 < nt ? s_u.seg.pref[q + 1] : tot;
-      const uint32_t pend = p0 + per < tot ? p0 + per : tot;
-      uint64_t src[ITEMS];
-#pragma unroll
-      for (int u = 0; u < ITEMS; ++u) {
-        const uint32_t p = p0 + u;
-        src[u] = 0;
-        if (p < pend) {
-          while (p >= qnext) {  // zero-length segments share their start
-            ++q;
-            qnext = q + 1 < nt ? s_u.seg.pref[q + 1] : tot;
-          }
-          src[u] = bs + (uint64_t)(c + q) * m.tile2 + s_u.seg.o0[q] + (p - s_u.seg.pref[q]);
-        }
-      }
-      KeyT v[ITEMS];
-#pragma unroll
-      for (int u = 0; u < ITEMS; ++u)
-        if (p0 + u < pend) v[u] = in[src[u]];
-#pragma unroll
-      for (int u = 0; u < ITEMS; ++u)
-        if (p0 + u < pend) B[mm + p0 + u] = v[u];
-    }
-    mm += tot;
-    __syncthreads();
-  }
-  const bool overflow = mm > (uint32_t)CAP;
-  if (overflow && tid == 0) atomicOr(err, ERR_OVERFLOW);
-  const uint32_t mcnt = overflow ? 0 : mm;
-  __syncthreads();
-
-  // LDS partition by the next b3 bits (unstable), then sort each small bin
   const uint32_t nd3 = 1u << m.d3.nbits;
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
+  for (uint32_t w = tid; w < 512 / 32; w += NT) s_mixed[w] = 0;
+  __syncthreads();
+  const uint32_t sb = s_sb, nsb = m.nb1 * m.nd2;
+  if (sb >= nsb) return;
+  const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
+  const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
+
   KeyT keys[ITEMS];
+  uint32_t mine;
+  const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT>(
+      in2, row0, row0 + m.ngroups, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2,
+      s_u.seg.pref, s_u.seg.o0, s_wave, &s_total, keys, mine);
+  if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
+  const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
+
+  // split by the next b3 bits into B (unstable, LDS atomics)
   uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t idx = tid + i * NT;
-    if (idx < mcnt) {
-      keys[i] = B[idx];
-      dig[i] = digit_eval(m.d3, keys[i]);
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      dig[u] = digit_eval(m.d3, keys[u]);
+      rank[u] = atomicAdd(&s_h[dig[u]], 1u);
     }
   }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if (tid + i * NT < mcnt) rank[i] = atomicAdd(&s_h[dig[i]], 1u);
   __syncthreads();
   scan_lds<NT>(s_h, nd3, s_wave, &s_total);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if (tid + i * NT < mcnt) B[s_h[dig[i]] + rank[i]] = keys[i];
+  for (int u = 0; u < ITEMS; ++u)
+    if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
   __syncthreads();
+
   // Order each bin by counting: a key's place is the number of keys of its
-  // bin that are smaller, plus the equal ones stored before it.  All reads are
-  // independent (no chain of dependent moves), and copies of one k-mer keep
-  // their relative order.  Bins whose keys are all equal stay as they are; a
-  // mixed bin longer than RANK_MAX flags the job, which is then recounted on
-  // the LSD path.
+  // bin that are smaller, plus the equal ones stored before it; reads are
+  // independent and copies of one k-mer keep their order.  Bins whose keys
+  // are all equal are already in order; a mixed bin longer than RANK_MAX flags
+  // the job, which is then recounted on the LSD path.
   constexpr uint32_t RANK_MAX = 128;
-  // bins whose keys are all equal (copies of one k-mer) are already in order
-  __shared__ uint32_t s_mixed[1024 / 32];
-  for (uint32_t w = tid; w < 1024 / 32; w += NT) s_mixed[w] = 0;
-  __syncthreads();
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
     const uint32_t i = tid + q * NT;
@@ -784,7 +818,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
         }
         newpos[q] = lo + r;
       } else {
-        bad = true;  // a long bin of different keys
+        bad = true;
       }
     }
   }
@@ -796,8 +830,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
   __syncthreads();
 
   // run-length reduce inside LDS
-  uint32_t heads = 0;
-  uint32_t hmask = 0;
+  uint32_t heads = 0, hmask = 0;
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
     const uint32_t i = tid * ITEMS + q;
@@ -807,6 +840,7 @@ __global__ __launch_bounds__(NT) void k_final(const KeyT *in, KeyT *out_keys, ui
   }
   uint32_t o = scan_block<NT>(heads, s_wave, &s_total);
   const uint32_t H = s_total;
+  uint16_t *s_hp = s_u.hp;
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q)
     if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
@@ -905,33 +939,39 @@ void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const 
     k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
 }
 
-void launch_bucket_tiles(int kw, const MsdArgs &m, const Workspace &ws, hipStream_t st) {
-  k_bucket_tiles<<<1, 256, 0, st>>>(ws.hist, m.nb1, (uint32_t)msd_tile(kw), m.tile_start);
+void launch_l1(int kw, const ReadsDesc &rd, const MsdArgs &m, void *out1, const Workspace &ws,
+               hipStream_t st) {
+  if (!m.ntiles1) return;
+  if (kw == 1)
+    k_l1<uint64_t, MSD_NT, MSD_ITEMS_1><<<m.ntiles1, MSD_NT, 0, st>>>(rd, m, (uint64_t *)out1, ws.n_dev,
+                                                                       ws.err);
+  else
+    k_l1<K2, MSD_NT, MSD_ITEMS_2><<<m.ntiles1, MSD_NT, 0, st>>>(rd, m, (K2 *)out1, ws.n_dev, ws.err);
 }
 
-void launch_part2(int kw, const void *in, void *out, const MsdArgs &m, const Workspace &ws,
-                  uint32_t grid, hipStream_t st) {
+void launch_l2(int kw, const void *in1, void *out2, const MsdArgs &m, const Workspace &ws,
+               hipStream_t st) {
+  const uint32_t grid = m.nb1 * m.ngroups;
   if (!grid) return;
   if (kw == 1)
-    k_part2<uint64_t, MSD_NT, MSD_ITEMS_1><<<grid, MSD_NT, 0, st>>>(
-        (const uint64_t *)in, (uint64_t *)out, m, ws.digit_start, ws.n_dev);
+    k_l2<uint64_t, MSD_NT, MSD_ITEMS_1><<<grid, MSD_NT, 0, st>>>((const uint64_t *)in1,
+                                                                  (uint64_t *)out2, m, ws.err);
   else
-    k_part2<K2, MSD_NT, MSD_ITEMS_2><<<grid, MSD_NT, 0, st>>>((const K2 *)in, (K2 *)out, m,
-                                                                    ws.digit_start, ws.n_dev);
+    k_l2<K2, MSD_NT, MSD_ITEMS_2><<<grid, MSD_NT, 0, st>>>((const K2 *)in1, (K2 *)out2, m, ws.err);
 }
 
-void launch_final(int kw, const void *in, void *out_keys, const MsdArgs &m, const Workspace &ws,
-                  uint32_t epoch, hipStream_t st) {
-  const uint32_t nsb = m.nb1 << m.b2;
+void launch_fin(int kw, const void *in2, void *out_keys, const MsdArgs &m, const Workspace &ws,
+                uint32_t epoch, hipStream_t st) {
+  const uint32_t nsb = m.nb1 * m.nd2;
   uint32_t *ticket = ws.tickets + MAX_PASSES + 1;
   if (kw == 1)
-    k_final<uint64_t, FIN_NT, FIN_ITEMS_1><<<nsb, FIN_NT, 0, st>>>(
-        (const uint64_t *)in, (uint64_t *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags,
-        ticket, epoch, ws.err);
+    k_fin<uint64_t, FIN_NT, FIN_ITEMS_1><<<nsb, FIN_NT, 0, st>>>(
+        (const uint64_t *)in2, (uint64_t *)out_keys, ws.counts, m, ws.n_dev, ws.flags, ticket, epoch,
+        ws.err);
   else
-    k_final<K2, FIN_NT, FIN_ITEMS_2><<<nsb, FIN_NT, 0, st>>>(
-        (const K2 *)in, (K2 *)out_keys, ws.counts, m, ws.digit_start, ws.n_dev, ws.flags, ticket,
-        epoch, ws.err);
+    k_fin<K2, FIN_NT, FIN_ITEMS_2><<<nsb, FIN_NT, 0, st>>>((const K2 *)in2, (K2 *)out_keys, ws.counts,
+                                                           m, ws.n_dev, ws.flags, ticket, epoch,
+                                                           ws.err);
 }
 
 }  // namespace dk

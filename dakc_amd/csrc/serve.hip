@@ -51,8 +51,8 @@ static size_t slots_footprint(const ServeState &s) {
   for (auto &sl : s.slots)
     for (const DevBuf *d : {&sl->ws.hist, &sl->ws.dstart, &sl->ws.ndev, &sl->ws.err, &sl->ws.tickets,
                             &sl->ws.flags, &sl->ws.rle_flags, &sl->ws.keys_a, &sl->ws.keys_b,
-                            &sl->ws.counts, &sl->ws.binoff, &sl->ws.reads, &sl->ws.msd_ts,
-                            &sl->ws.msd_off})
+                            &sl->ws.counts, &sl->ws.binoff, &sl->ws.reads, &sl->ws.msd_off1,
+                            &sl->ws.msd_off2})
       b += d->cap;
   return b;
 }
@@ -62,7 +62,7 @@ static size_t slot_growth(const Slot &sl, const Plan &pl, bool need_reads) {
   const size_t need = job_device_bytes(pl) - (need_reads ? 0 : pl.nbytes);
   size_t have = 0;
   for (const DevBuf *d : {&sl.ws.flags, &sl.ws.rle_flags, &sl.ws.keys_a, &sl.ws.keys_b,
-                          &sl.ws.counts, &sl.ws.reads, &sl.ws.msd_off})
+                          &sl.ws.counts, &sl.ws.reads, &sl.ws.msd_off2})
     have += d->cap;
   return need > have ? need - have : 0;
 }
@@ -144,7 +144,7 @@ static void scheduler(ServeState *s) {
             sl->ws.reads.release();
             sl->ws.flags.release();
             sl->ws.rle_flags.release();
-            sl->ws.msd_off.release();
+            sl->ws.msd_off2.release();
           }
         if (slots_footprint(*s) + best > s->budget) break;  // wait for running jobs
       }

@@ -38,6 +38,20 @@ WORKLOADS = {
 }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/traffic.json), or None."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    tag = {"genome": "genome_k31", "iid": "iid_k31"}.get(workload)
+    data = json.loads(f.read_text()).get(tag, {}).get("kernels", {})
+    for name, v in data.items():
+        if name.startswith(f"dk::{kernel}<"):
+            return v["traffic_bytes"]
+    return None
+
+
 def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
     import oracle
     from dakc_amd import _lib
@@ -68,6 +82,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--algo", default="auto", choices=["auto", "lsd"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also copy the read buffer device-to-device once (a plain copy kernel "
+                         "of known size, the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU's count; --gpus {a.gpus} is not supported in this build "
@@ -86,6 +103,12 @@ def main() -> int:
     kc.generate_device(p, dreads, nbytes)
     print(f"[bench] generated {nbytes / 1e9:.2f} GB of reads on the device", file=sys.stderr,
           flush=True)
+
+    if a.copy_probe:
+        dcopy = kc.device_alloc(nbytes)
+        kc.d2d(dcopy, dreads, nbytes)
+        kc.device_free(dcopy)
+        print(f"[bench] copy probe: {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
     for i in range(a.warmup):
         t = kc.count_device(dreads, nbytes)
@@ -116,13 +139,14 @@ def main() -> int:
     # in = record bytes, n = k-mers, d = distinct, w = key bytes, c = 4 (count)
     kb, inb = 8 * (1 if w["k"] <= 32 else 2), nbytes
     kernels = {}
-    kernels["k_hist_reads"] = (ms["hist"], inb)
-    kernels["k_sweep<from_reads>"] = (ms["pass0"], inb + n_kmers * kb)
     if path == 2:
-        kernels["k_part2"] = (ms["part2"], 2 * n_kmers * kb)
-        kernels["k_final"] = (ms["final"], n_kmers * kb + d * (kb + 4))
-        b_alg = 2 * inb + 4 * n_kmers * kb + d * (kb + 4)
+        kernels["k_l1"] = (ms["pass0"], inb + n_kmers * kb)
+        kernels["k_l2"] = (ms["part2"], 2 * n_kmers * kb)
+        kernels["k_fin"] = (ms["final"], n_kmers * kb + d * (kb + 4))
+        b_alg = inb + 4 * n_kmers * kb + d * (kb + 4)
     else:
+        kernels["k_hist_reads"] = (ms["hist"], inb)
+        kernels["k_sweep<from_reads>"] = (ms["pass0"], inb + n_kmers * kb)
         key_pass_ms = sweep_sum[1:npasses] / a.steps
         kernels["k_sweep<keys>"] = (float(key_pass_ms.mean()) if npasses > 1 else 0.0, 2 * n_kmers * kb)
         kernels["k_rle"] = (ms["rle"], n_kmers * kb + d * (kb + 4))
@@ -132,6 +156,7 @@ def main() -> int:
     dom = max(total_kernel_ms, key=total_kernel_ms.get)
     dom_ms, dom_bytes = kernels[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = measured_traffic(a.workload, dom)
     stages = {
         "path": "msd" if path == 2 else "lsd", "msd_reruns": reruns, "distinct": d,
         "kernels": {name: {"avg_ms": t_ms, "alg_bytes": nb_,
@@ -159,7 +184,7 @@ def main() -> int:
                                f"k={w['k']}", "k": w["k"], "kmers_per_step": n_kmers,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
     }

@@ -29,7 +29,7 @@ EXPORTS = [
     "dakc_count_device", "dakc_last_stats", "dakc_count_batch", "dakc_submit", "dakc_poll",
     "dakc_wait", "dakc_table_free", "dakc_table_serialize", "dakc_gen_nbytes", "dakc_generate_reads",
     "dakc_generate_reads_device", "dakc_device_alloc", "dakc_device_free", "dakc_memcpy_h2d",
-    "dakc_memcpy_d2h",
+    "dakc_memcpy_d2h", "dakc_memcpy_d2d",
 ]
 
 
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
         "dakc_device_free": (C.c_int, [V, V]),
         "dakc_memcpy_h2d": (C.c_int, [V, V, V, C.c_uint64]),
         "dakc_memcpy_d2h": (C.c_int, [V, V, V, C.c_uint64]),
+        "dakc_memcpy_d2d": (C.c_int, [V, V, V, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

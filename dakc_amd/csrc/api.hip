@@ -632,6 +632,13 @@ extern "C" int dakc_memcpy_d2h(dakc_ctx *ctx, void *dst, const void *src, uint64
   return hipMemcpy(dst, src, n, hipMemcpyDeviceToHost) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
 }
 
+extern "C" int dakc_memcpy_d2d(dakc_ctx *ctx, void *dst, const void *src, uint64_t n) {
+  if (!ctx) return DAKC_E_INVAL;
+  hipSetDevice(ctx->device);
+  if (hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) return DAKC_E_DEVICE;
+  return hipStreamSynchronize(ctx->stream) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
 // ---------------------------------------------------------- generator --
 
 namespace dk {

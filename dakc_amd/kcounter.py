@@ -201,6 +201,10 @@ class KmerCounter:
                                              C.c_void_p(dev_ptr), nbytes), "dakc_memcpy_d2h")
         return out
 
+    def d2d(self, dst: int, src: int, nbytes: int) -> None:
+        check(self._libh.dakc_memcpy_d2d(self._ctx, C.c_void_p(dst), C.c_void_p(src), nbytes),
+              "dakc_memcpy_d2d")
+
     def h2d(self, dev_ptr: int, arr: np.ndarray) -> None:
         arr = np.ascontiguousarray(arr)
         if arr.nbytes:

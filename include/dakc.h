@@ -165,6 +165,7 @@ int dakc_device_alloc(dakc_ctx *ctx, uint64_t nbytes, void **dptr);
 int dakc_device_free(dakc_ctx *ctx, void *dptr);
 int dakc_memcpy_h2d(dakc_ctx *ctx, void *dst, const void *src, uint64_t nbytes);
 int dakc_memcpy_d2h(dakc_ctx *ctx, void *dst, const void *src, uint64_t nbytes);
+int dakc_memcpy_d2d(dakc_ctx *ctx, void *dst, const void *src, uint64_t nbytes);
 
 #ifdef __cplusplus
 }

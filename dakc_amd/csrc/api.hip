@@ -237,7 +237,9 @@ int plan_choose_path(Plan *pl, int algo) {
   p.msd = 0;
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
-  const uint64_t cap3 = (uint64_t)fin_cap(p.kw), cap2 = (uint64_t)msd_tile(p.kw);
+  const char *fe = std::getenv("DAKC_FIN_NT");
+  const uint32_t fin_nt = fe && std::atoi(fe) == 512 ? 512u : 256u;
+  const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap2 = (uint64_t)msd_tile(p.kw);
   const uint64_t target = cap3 * 3 / 5;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
@@ -259,7 +261,8 @@ int plan_choose_path(Plan *pl, int algo) {
   if ((uint64_t)rt * p.S > cap2) return 0;
   p.b1 = b1;
   p.b2 = b2;
-  p.b3 = std::min<uint32_t>(9, bits - b1 - b2);
+  p.b3 = std::min<uint32_t>(fin_nt == 512 ? 10 : 9, bits - b1 - b2);
+  p.fin_nt = fin_nt;
   p.nb1 = owners << b1;
   p.nd2 = 1u << b2;
   p.nsb = p.nb1 * p.nd2;
@@ -320,6 +323,7 @@ static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd0, int num_
   m.G = pl.G;
   m.ngroups = pl.ngroups;
   m.cap2 = pl.cap2;
+  m.fin_nt = pl.fin_nt;
   m.off1 = (uint16_t *)ws.msd_off1.p;
   m.off2 = (uint16_t *)ws.msd_off2.p;
   launch_fill_u64(w.n_dev, 0, 2, st);

@@ -38,7 +38,7 @@ struct Plan {
   // are counted), second-level b2, in-LDS b3; L1 tiles of RT1 records
   int msd = 0;
   uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1;
-  uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0;
+  uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path

@@ -661,7 +661,10 @@ __global__ __launch_bounds__(NT) void k_l2(const KeyT *in1, KeyT *out2, MsdArgs 
   __shared__ uint32_t s_wave[NT / 64], s_total;
   const uint32_t tid = threadIdx.x;
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t b = wg / m.ngroups, g = wg - b * m.ngroups;
+  // consecutive workgroups (one XCD's range) take consecutive buckets of the
+  // same group: their segments are neighbours in every L1 tile, so the cache
+  // lines they share are still in that XCD's L2
+  const uint32_t g = wg / m.nb1, b = wg - g * m.nb1;
   const uint32_t t0 = g * m.G;
   const uint32_t nt = m.ntiles1 - t0 < m.G ? m.ntiles1 - t0 : m.G;
   const uint16_t *row0 = m.off1 + (uint64_t)b * m.ntiles1 + t0;
@@ -731,7 +734,7 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
-template <typename KeyT, int NT, int ITEMS>
+template <typename KeyT, int NT, int ITEMS, int NB3>
 __global__ __launch_bounds__(NT) void k_fin(const KeyT *in2, KeyT *out_keys, uint32_t *out_counts,
                                             MsdArgs m, uint64_t *n_dev, uint64_t *flags,
                                             uint32_t *ticket, uint32_t epoch, uint32_t *err) {
@@ -745,15 +748,15 @@ __global__ __launch_bounds__(NT) void k_fin(const KeyT *in2, KeyT *out_keys, uin
     } seg;
     uint16_t hp[CAP];
   } s_u;
-  __shared__ uint32_t s_h[512 + 1];
-  __shared__ uint32_t s_mixed[512 / 32];
+  __shared__ uint32_t s_h[NB3 + 1];
+  __shared__ uint32_t s_mixed[NB3 / 32];
   __shared__ uint32_t s_wave[NT / 64], s_total, s_sb;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   if (tid == 0) s_sb = atomicAdd(ticket, 1u);
   const uint32_t nd3 = 1u << m.d3.nbits;
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
-  for (uint32_t w = tid; w < 512 / 32; w += NT) s_mixed[w] = 0;
+  for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
   __syncthreads();
   const uint32_t sb = s_sb, nsb = m.nb1 * m.nd2;
   if (sb >= nsb) return;
@@ -964,14 +967,17 @@ void launch_fin(int kw, const void *in2, void *out_keys, const MsdArgs &m, const
                 uint32_t epoch, hipStream_t st) {
   const uint32_t nsb = m.nb1 * m.nd2;
   uint32_t *ticket = ws.tickets + MAX_PASSES + 1;
-  if (kw == 1)
-    k_fin<uint64_t, FIN_NT, FIN_ITEMS_1><<<nsb, FIN_NT, 0, st>>>(
-        (const uint64_t *)in2, (uint64_t *)out_keys, ws.counts, m, ws.n_dev, ws.flags, ticket, epoch,
-        ws.err);
-  else
-    k_fin<K2, FIN_NT, FIN_ITEMS_2><<<nsb, FIN_NT, 0, st>>>((const K2 *)in2, (K2 *)out_keys, ws.counts,
-                                                           m, ws.n_dev, ws.flags, ticket, epoch,
-                                                           ws.err);
+#define DK_FIN(KT, NT_, IT_, NB_)                                                                  \
+  k_fin<KT, NT_, IT_, NB_><<<nsb, NT_, 0, st>>>((const KT *)in2, (KT *)out_keys, ws.counts, m,   \
+                                                ws.n_dev, ws.flags, ticket, epoch, ws.err)
+  if (kw == 1) {
+    if (m.fin_nt == 512) DK_FIN(uint64_t, 512, FIN_ITEMS_1, 1024);
+    else DK_FIN(uint64_t, 256, FIN_ITEMS_1, 512);
+  } else {
+    if (m.fin_nt == 512) DK_FIN(K2, 512, FIN_ITEMS_2, 1024);
+    else DK_FIN(K2, 256, FIN_ITEMS_2, 512);
+  }
+#undef DK_FIN
 }
 
 }  // namespace dk

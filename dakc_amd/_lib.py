@@ -29,7 +29,9 @@ EXPORTS = [
     "dakc_count_device", "dakc_last_stats", "dakc_count_batch", "dakc_submit", "dakc_poll",
     "dakc_wait", "dakc_table_free", "dakc_table_serialize", "dakc_gen_nbytes", "dakc_generate_reads",
     "dakc_generate_reads_device", "dakc_device_alloc", "dakc_device_free", "dakc_memcpy_h2d",
-    "dakc_memcpy_d2h", "dakc_memcpy_d2d",
+    "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_create", "dakc_batch_run",
+    "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
+    "dakc_tuning_load",
 ]
 
 
@@ -117,6 +119,13 @@ def load() -> C.CDLL:
         "dakc_memcpy_h2d": (C.c_int, [V, V, V, C.c_uint64]),
         "dakc_memcpy_d2h": (C.c_int, [V, V, V, C.c_uint64]),
         "dakc_memcpy_d2d": (C.c_int, [V, V, V, C.c_uint64]),
+        "dakc_batch_create": (C.c_int, [V, P(Job), C.c_size_t, P(V)]),
+        "dakc_batch_run": (C.c_int, [V, P(C.c_double)]),
+        "dakc_batch_tables": (C.c_int, [V, P(Table)]),
+        "dakc_batch_destroy": (C.c_int, [V]),
+        "dakc_autotune": (C.c_int, [V, P(Job), C.c_int, P(C.c_double)]),
+        "dakc_tuning_save": (C.c_int, [V, C.c_char_p]),
+        "dakc_tuning_load": (C.c_int, [V, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -80,7 +80,7 @@ void DevBuf::release() {
 
 JobWs::~JobWs() {
   for (auto *b : {&hist, &dstart, &ndev, &err, &tickets, &flags, &rle_flags, &keys_a, &keys_b,
-                  &counts, &binoff, &reads, &msd_off1, &msd_off2})
+                  &counts, &binoff, &reads, &msd_off1, &msd_off2, &arena, &meta, &desc})
     b->release();
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
@@ -115,36 +115,33 @@ Workspace JobWs::view() const {
   w.keys_b = keys_b.p;
   w.counts = (uint32_t *)counts.p;
   w.bin_offsets = (uint64_t *)binoff.p;
+  w.key_cap = 0;
   return w;
 }
 
+size_t msd_job_bytes(const Plan &pl, bool copy_reads);
+
 size_t job_device_bytes(const Plan &pl) {
+  if (pl.msd) return msd_job_bytes(pl, true) + 8192;
   const size_t kb = (size_t)pl.kw * 8;
   size_t b = 0;
   b += 2 * (size_t)MAX_PASSES * 256 * 8 + 64 + 4096;
   b += (size_t)std::max<uint64_t>(pl.tiles0, pl.tiles_keys) * 256 * 8 + (size_t)pl.tiles_keys * 8;
   b += 2 * (size_t)pl.nmax * kb + (size_t)pl.nmax * 4;
   b += (size_t)pl.nbytes + 256;
-  if (pl.msd) {
-    b += ((size_t)pl.nb1 + 1) * pl.ntiles1 * 2 + (size_t)pl.nb1 * (pl.nd2 + 1) * pl.ngroups * 2;
-    b += (size_t)pl.nsb * 8;
-    const size_t out1 = (size_t)pl.ntiles1 * pl.tcap1, out2 = (size_t)pl.nb1 * pl.ngroups * pl.cap2;
-    if (out1 > pl.nmax) b += (out1 - pl.nmax) * kb;
-    if (out2 > pl.nmax) b += (out2 - pl.nmax) * kb;
-  }
   return b;
 }
 
 int JobWs::reserve(const Plan &pl, bool need_reads) {
   int s;
   const size_t kb = (size_t)pl.kw * 8;
+  if (pl.msd) return DAKC_OK;  // msd_stage sizes the arena for the job
   if ((s = hist.ensure((size_t)MAX_PASSES * 256 * 8, stream, false))) return s;
   if ((s = dstart.ensure((size_t)MAX_PASSES * 256 * 8, stream, false))) return s;
   if ((s = ndev.ensure(64, stream, false))) return s;
   if ((s = err.ensure(64, stream, false))) return s;
   if ((s = tickets.ensure((MAX_PASSES + 2) * 4, stream, false))) return s;
   size_t ft = std::max<uint64_t>(pl.tiles0, pl.tiles_keys);
-  if (pl.msd) ft = std::max<size_t>(ft, ((size_t)pl.nsb + 255) / 256);
   const size_t fl_before = flags.cap, rf_before = rle_flags.cap;
   if ((s = flags.ensure(std::max<size_t>(ft, 1) * 256 * 8, stream, true))) return s;
   if ((s = rle_flags.ensure(std::max<size_t>(pl.tiles_keys, 1) * 8, stream, true))) return s;
@@ -154,21 +151,11 @@ int JobWs::reserve(const Plan &pl, bool need_reads) {
     launch_fill_u64((uint64_t *)rle_flags.p, 0, rle_flags.cap / 8, stream);
     epoch = 0;
   }
-  size_t na = pl.nmax, nbk = pl.nmax;
-  if (pl.msd) {
-    na = std::max<size_t>(na, (size_t)pl.ntiles1 * pl.tcap1);
-    nbk = std::max<size_t>(nbk, (size_t)pl.nb1 * pl.ngroups * pl.cap2);
-  }
-  if ((s = keys_a.ensure(std::max<size_t>(na, 1) * kb + 256, stream, false))) return s;
-  if ((s = keys_b.ensure(std::max<size_t>(nbk, 1) * kb + 256, stream, false))) return s;
+  if ((s = keys_a.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
+  if ((s = keys_b.ensure(std::max<size_t>(pl.nmax, 1) * kb + 256, stream, false))) return s;
   if ((s = counts.ensure(std::max<size_t>(pl.nmax, 1) * 4 + 256, stream, false))) return s;
   if ((s = binoff.ensure(((size_t)pl.num_pes + 1) * 8 + 64, stream, false))) return s;
   if (need_reads && (s = reads.ensure(pl.nbytes + 256, stream, false))) return s;
-  if (pl.msd) {
-    if ((s = msd_off1.ensure(((size_t)pl.nb1 + 1) * pl.ntiles1 * 2 + 256, stream, false))) return s;
-    if ((s = msd_off2.ensure((size_t)pl.nb1 * (pl.nd2 + 1) * pl.ngroups * 2 + 256, stream, false)))
-      return s;
-  }
   return DAKC_OK;
 }
 
@@ -226,8 +213,8 @@ int make_plan(const dakc_job *job, Plan *pl) {
   return DAKC_OK;
 }
 
-// MSD geometry.  A final sub-bucket should hold at most 60% of what one
-// final workgroup sorts in LDS, which fixes b1 + b2; b1 <= 10 (1024 first-level
+// MSD geometry.  A final sub-bucket should hold at most tune_fill_pct (60 by
+// default) percent of what one final workgroup sorts in LDS, which fixes b1 + b2; b1 <= 10 (1024 first-level
 // buckets, fewer key bits when owner bins share them) and b2 <= 10.  L1 tiles
 // take as many records as fill 15360 (8192) slots; an L2 group gathers G
 // consecutive L1 tiles of a bucket, G sized so a group fills 75% of an L2
@@ -237,10 +224,9 @@ int plan_choose_path(Plan *pl, int algo) {
   p.msd = 0;
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
-  const char *fe = std::getenv("DAKC_FIN_NT");
-  const uint32_t fin_nt = fe && std::atoi(fe) == 512 ? 512u : 256u;
+  const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : 512u;
   const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap2 = (uint64_t)msd_tile(p.kw);
-  const uint64_t target = cap3 * 3 / 5;
+  const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 60) / 100;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
   const uint32_t owners = p.owner_digit ? p.num_pes : 1;
@@ -301,52 +287,208 @@ ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
   return rd;
 }
 
-// Enqueue the whole count of one job on ws.stream.  The sorted table ends in
-// *table_keys (one of the two key buffers) and ws.counts.
-static int enqueue_msd(JobWs &ws, const Plan &pl, const ReadsDesc &rd0, int num_cus, bool events,
-                       void **table_keys) {
-  (void)num_cus;
-  hipStream_t st = ws.stream;
-  Workspace w = ws.view();
-  ReadsDesc rd = rd0;  // L1 tiles hold RT1 records
-  rd.RT = pl.RT1;
-  rd.wpr = pl.wpr1;
-  const uint32_t bits = 2 * (uint32_t)pl.k;
-  MsdArgs m;
-  m.d1 = {DIG_MSD, bits - pl.b1, pl.b1, pl.owner_digit ? pl.num_pes : 1u};
-  m.d2 = {DIG_BITS, bits - pl.b1 - pl.b2, pl.b2, 1};
-  m.d3 = {DIG_BITS, bits - pl.b1 - pl.b2 - pl.b3, pl.b3, 1};
-  m.nb1 = pl.nb1;
-  m.nd2 = pl.nd2;
-  m.ntiles1 = pl.ntiles1;
-  m.tcap1 = pl.tcap1;
-  m.G = pl.G;
-  m.ngroups = pl.ngroups;
-  m.cap2 = pl.cap2;
-  m.fin_nt = pl.fin_nt;
-  m.off1 = (uint16_t *)ws.msd_off1.p;
-  m.off2 = (uint16_t *)ws.msd_off2.p;
-  launch_fill_u64(w.n_dev, 0, 2, st);
-  launch_fill_u32(w.err, 0, 1, st);
-  launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
-  if (events) hipEventRecord(ws.ev[0], st);
-  launch_l1(pl.kw, rd, m, w.keys_a, w, st);
-  if (events) hipEventRecord(ws.ev[1], st);
-  launch_l2(pl.kw, w.keys_a, w.keys_b, m, w, st);
-  if (events) hipEventRecord(ws.ev[2], st);
-  launch_fin(pl.kw, w.keys_b, w.keys_a, m, w, ws.next_epoch(), st);
-  if (pl.owner_digit) launch_bin_offsets(pl.kw, w.keys_a, pl.num_pes, w, st);
-  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
-  *table_keys = w.keys_a;
-  return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+// Enqueue the whole count of one job on ws.stream; ws.loc says where the
+// table and the status words will be.
+// ------------------------------------------------------ multi-job MSD --
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t META_STRIDE = 32;  // per job: u64 n, u64 d, u32 err
+
+// Arena layout: all jobs' look-back flags first (one fill clears them), then
+// per job its reads copy (host input only), L1 tiles / table keys, L2 regions,
+// counts, offsets and owner-bin offsets.
+void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads) {
+  const size_t nj = plans.size();
+  b.plans = plans;
+  b.kw = nj ? plans[0].kw : 1;
+  b.fin_nt = nj ? plans[0].fin_nt : 256;
+  for (auto *v : {&b.off_reads, &b.off_out1, &b.off_out2, &b.off_counts, &b.off_off1, &b.off_off2,
+                  &b.off_flags, &b.off_binoff})
+    v->assign(nj, 0);
+  b.pre1.assign(nj + 1, 0);
+  b.pre2.assign(nj + 1, 0);
+  b.pre3.assign(nj + 1, 0);
+  size_t o = 0;
+  b.flags_off = 0;
+  for (size_t j = 0; j < nj; ++j) {
+    b.off_flags[j] = o;
+    o += (size_t)plans[j].nsb * 8;
+  }
+  b.flags_words = o / 8;
+  o = al256(o);
+  for (size_t j = 0; j < nj; ++j) {
+    const Plan &p = plans[j];
+    const size_t kb = (size_t)p.kw * 8;
+    if (copy_reads[j]) {
+      b.off_reads[j] = o;
+      o = al256(o + p.nbytes + 4);
+    } else {
+      b.off_reads[j] = (size_t)-1;
+    }
+    b.off_out1[j] = o;
+    o = al256(o + std::max<size_t>((size_t)p.ntiles1 * p.tcap1, p.nmax) * kb);
+    b.off_out2[j] = o;
+    o = al256(o + (size_t)p.nb1 * p.ngroups * p.cap2 * kb);
+    b.off_counts[j] = o;
+    o = al256(o + std::max<size_t>(p.nmax, 1) * 4);
+    b.off_off1[j] = o;
+    o = al256(o + ((size_t)p.nb1 + 1) * p.ntiles1 * 2);
+    b.off_off2[j] = o;
+    o = al256(o + (size_t)p.nb1 * (p.nd2 + 1) * p.ngroups * 2);
+    b.off_binoff[j] = o;
+    o = al256(o + ((size_t)p.num_pes + 1) * 8);
+    b.pre1[j + 1] = b.pre1[j] + p.ntiles1;
+    b.pre2[j + 1] = b.pre2[j] + p.nb1 * p.ngroups;
+    b.pre3[j + 1] = b.pre3[j] + p.nsb;
+  }
+  b.arena_bytes = o;
+  b.meta_bytes = al256(nj * META_STRIDE);
+  b.desc_bytes = al256(nj * sizeof(JobDesc)) + al256(3 * (nj + 1) * 4) + 256;
 }
 
-int enqueue_count(JobWs &ws, const Plan &pl, const uint8_t *dreads, int num_cus, bool events,
-                  void **table_keys) {
+size_t msd_job_bytes(const Plan &pl, bool copy_reads) {
+  MsdBatch b;
+  msd_layout(b, {pl}, {copy_reads});
+  return b.arena_bytes + b.meta_bytes + b.desc_bytes;
+}
+
+int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs) {
+  const size_t nj = b.plans.size();
+  int s;
+  if ((s = ws.arena.ensure(std::max<size_t>(b.arena_bytes, 256), ws.stream, false))) return s;
+  if ((s = ws.meta.ensure(std::max<size_t>(b.meta_bytes, 256), ws.stream, false))) return s;
+  if ((s = ws.desc.ensure(b.desc_bytes, ws.stream, false))) return s;
+  uint8_t *ar = (uint8_t *)ws.arena.p;
+  uint8_t *mt = (uint8_t *)ws.meta.p;
+  uint8_t *dd = (uint8_t *)ws.desc.p;
+  std::vector<JobDesc> desc(nj);
+  b.locs.assign(nj, TableLoc());
+  for (size_t j = 0; j < nj; ++j) {
+    const Plan &p = b.plans[j];
+    const uint8_t *dreads = (const uint8_t *)jobs[j]->reads;
+    if (b.off_reads[j] != (size_t)-1) {
+      if (p.nbytes && hipMemcpyAsync(ar + b.off_reads[j], jobs[j]->reads, p.nbytes,
+                                     hipMemcpyHostToDevice, ws.stream) != hipSuccess)
+        return DAKC_E_DEVICE;
+      dreads = ar + b.off_reads[j];
+    }
+    JobDesc &D = desc[j];
+    std::memset(&D, 0, sizeof(D));
+    D.rd = plan_reads(p, dreads);
+    D.rd.RT = p.RT1;
+    D.rd.wpr = p.wpr1;
+    const uint32_t bits = 2 * (uint32_t)p.k;
+    MsdArgs &m = D.m;
+    m.d1 = {DIG_MSD, bits - p.b1, p.b1, p.owner_digit ? p.num_pes : 1u};
+    m.d2 = {DIG_BITS, bits - p.b1 - p.b2, p.b2, 1};
+    m.d3 = {DIG_BITS, bits - p.b1 - p.b2 - p.b3, p.b3, 1};
+    m.nb1 = p.nb1;
+    m.nd2 = p.nd2;
+    m.ntiles1 = p.ntiles1;
+    m.tcap1 = p.tcap1;
+    m.G = p.G;
+    m.ngroups = p.ngroups;
+    m.cap2 = p.cap2;
+    m.fin_nt = p.fin_nt;
+    m.off1 = (uint16_t *)(ar + b.off_off1[j]);
+    m.off2 = (uint16_t *)(ar + b.off_off2[j]);
+    D.out1 = ar + b.off_out1[j];
+    D.out2 = ar + b.off_out2[j];
+    D.tcounts = (uint32_t *)(ar + b.off_counts[j]);
+    D.nd = (uint64_t *)(mt + j * META_STRIDE);
+    D.err = (uint32_t *)(mt + j * META_STRIDE + 16);
+    D.flags = (uint64_t *)(ar + b.off_flags[j]);
+    TableLoc &L = b.locs[j];
+    L.keys = D.out1;
+    L.counts = D.tcounts;
+    L.binoff = p.owner_digit ? (uint64_t *)(ar + b.off_binoff[j]) : nullptr;
+    L.nd = D.nd;
+    L.err = D.err;
+  }
+  // argument block: JobDesc[nj] | pre1 | pre2 | pre3 | ticket
+  std::vector<uint8_t> host(b.desc_bytes, 0);
+  std::memcpy(host.data(), desc.data(), nj * sizeof(JobDesc));
+  uint32_t *pre = (uint32_t *)(host.data() + al256(nj * sizeof(JobDesc)));
+  for (size_t j = 0; j <= nj; ++j) {
+    pre[j] = b.pre1[j];
+    pre[nj + 1 + j] = b.pre2[j];
+    pre[2 * (nj + 1) + j] = b.pre3[j];
+  }
+  if (hipMemcpyAsync(dd, host.data(), b.desc_bytes, hipMemcpyHostToDevice, ws.stream) != hipSuccess)
+    return DAKC_E_DEVICE;
+  // the staging vector goes out of scope: finish the copy first
+  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+  return DAKC_OK;
+}
+
+static BatchArgs batch_args(const JobWs &ws, const MsdBatch &b) {
+  const size_t nj = b.plans.size();
+  uint8_t *dd = (uint8_t *)ws.desc.p;
+  uint32_t *pre = (uint32_t *)(dd + al256(nj * sizeof(JobDesc)));
+  BatchArgs a;
+  a.jobs = (const JobDesc *)dd;
+  a.pre1 = pre;
+  a.pre2 = pre + (nj + 1);
+  a.pre3 = pre + 2 * (nj + 1);
+  a.njobs = (uint32_t)nj;
+  a.ticket = pre + 3 * (nj + 1);
+  return a;
+}
+
+// Fill kernels and the three levels; no host transfer, so a stream capture of
+// this sequence replays it exactly.
+void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
+  hipStream_t st = ws.stream;
+  const size_t nj = b.plans.size();
+  const BatchArgs a = batch_args(ws, b);
+  launch_fill_u64((uint64_t *)ws.meta.p, 0, b.meta_bytes / 8, st);
+  launch_fill_u64((uint64_t *)ws.arena.p + b.flags_off / 8, 0, b.flags_words, st);
+  launch_fill_u32(a.ticket, 0, 1, st);
+  if (events) hipEventRecord(ws.ev[0], st);
+  launch_msd_batch(b.kw, b.fin_nt, a, b.pre1[nj], 0, 0, st);
+  if (events) hipEventRecord(ws.ev[1], st);
+  launch_msd_batch(b.kw, b.fin_nt, a, 0, b.pre2[nj], 0, st);
+  if (events) hipEventRecord(ws.ev[2], st);
+  launch_msd_batch(b.kw, b.fin_nt, a, 0, 0, b.pre3[nj], st);
+  for (size_t j = 0; j < nj; ++j)
+    if (b.plans[j].owner_digit)
+      launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.locs[j].binoff,
+                            st);
+  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
+}
+
+int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, uint64_t *d) {
+  const uint8_t *m = (const uint8_t *)meta_host + j * META_STRIDE;
+  uint64_t nd[2];
+  uint32_t e;
+  std::memcpy(nd, m, 16);
+  std::memcpy(&e, m + 16, 4);
+  if (e & (ERR_SEPARATOR | ERR_NUL_RECORD)) return DAKC_E_FORMAT;
+  if (e & ERR_OVERFLOW) return STATUS_MSD_OVERFLOW;
+  *n = nd[0];
+  *d = b.plans[j].nmax ? nd[1] : 0;
+  return DAKC_OK;
+}
+
+int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t *dreads,
+                  int num_cus, bool events) {
+  if (pl.msd) {
+    msd_layout(ws.single, {pl}, {!job->reads_on_device});
+    int s = msd_stage(ws, ws.single, {job});
+    if (s) return s;
+    msd_enqueue(ws, ws.single, events);
+    ws.loc = ws.single.locs[0];
+    return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+  }
   hipStream_t st = ws.stream;
   Workspace w = ws.view();
+  w.key_cap = pl.nmax;  // reserve() sized keys_a, keys_b and counts for nmax keys
+  if (!job->reads_on_device && pl.nbytes) {
+    if (hipMemcpyAsync(ws.reads.p, job->reads, pl.nbytes, hipMemcpyHostToDevice, st) != hipSuccess)
+      return DAKC_E_DEVICE;
+    dreads = (const uint8_t *)ws.reads.p;
+  }
   const ReadsDesc rd = plan_reads(pl, dreads);
-  if (pl.msd) return enqueue_msd(ws, pl, rd, num_cus, events, table_keys);
   launch_fill_u64(w.hist, 0, (uint64_t)pl.npasses * 256, st);
   launch_fill_u64(w.n_dev, 0, 2, st);
   launch_fill_u32(w.err, 0, 1, st);
@@ -384,12 +526,22 @@ int enqueue_count(JobWs &ws, const Plan &pl, const uint8_t *dreads, int num_cus,
     if (pl.owner_digit) launch_bin_offsets(pl.kw, other, pl.num_pes, w, st);
   }
   if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
-  *table_keys = other;
+  ws.loc.keys = other;
+  ws.loc.counts = w.counts;
+  ws.loc.binoff = w.bin_offsets;
+  ws.loc.nd = w.n_dev;
+  ws.loc.err = w.err;
   return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
 }
 
 // After the stream has drained: read n, d and the error word.
 int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d) {
+  if (pl.msd) {
+    uint8_t meta[META_STRIDE];
+    if (hipMemcpy(meta, ws.loc.nd, META_STRIDE, hipMemcpyDeviceToHost) != hipSuccess)
+      return DAKC_E_DEVICE;
+    return msd_finish(ws.single, 0, meta, n, d);
+  }
   uint64_t nd[2] = {0, 0};
   uint32_t e = 0;
   if (hipMemcpy(nd, ws.ndev.p, 16, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
@@ -494,21 +646,13 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   Plan pl;
   int s = make_plan(job, &pl);
   if (s) return s;
-  plan_choose_path(&pl, c->opts.algo);
+  apply_tuning(c, &pl, c->opts.algo);
   JobWs &ws = c->ws;
-  const uint8_t *dreads = (const uint8_t *)job->reads;
-  void *tkeys = nullptr;
   uint64_t n = 0, d = 0;
   uint32_t reruns = 0;
   for (;;) {
     if ((s = ws.reserve(pl, !job->reads_on_device))) return s;
-    if (!job->reads_on_device && pl.nbytes && dreads == (const uint8_t *)job->reads) {
-      if (hipMemcpyAsync(ws.reads.p, job->reads, pl.nbytes, hipMemcpyHostToDevice, ws.stream) !=
-          hipSuccess)
-        return DAKC_E_DEVICE;
-      dreads = (const uint8_t *)ws.reads.p;
-    }
-    if ((s = enqueue_count(ws, pl, dreads, c->num_cus, true, &tkeys))) return s;
+    if ((s = enqueue_count(ws, pl, job, (const uint8_t *)job->reads, c->num_cus, true))) return s;
     if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
     s = finish_count(ws, pl, &n, &d);
     if (s == STATUS_MSD_OVERFLOW && pl.msd) {  // skewed input: count it on the LSD path
@@ -519,6 +663,7 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
     if (s) return s;
     break;
   }
+  const TableLoc loc = ws.loc;
   fill_stats(c->last, ws, pl, n, d);
   c->last.msd_reruns = reruns;
 
@@ -531,7 +676,7 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
     out->bin_offsets = (uint64_t *)std::malloc(((size_t)pl.num_pes + 1) * 8);
     if (!out->bin_offsets) return DAKC_E_NOMEM;
     if (d) {
-      if (hipMemcpy(out->bin_offsets, ws.binoff.p, ((size_t)pl.num_pes + 1) * 8,
+      if (hipMemcpy(out->bin_offsets, loc.binoff, ((size_t)pl.num_pes + 1) * 8,
                     hipMemcpyDeviceToHost) != hipSuccess)
         return DAKC_E_DEVICE;
     } else {
@@ -539,8 +684,8 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
     }
   }
   if (!to_host) {
-    out->keys = (uint64_t *)tkeys;
-    out->counts = (uint32_t *)ws.counts.p;
+    out->keys = (uint64_t *)loc.keys;
+    out->counts = loc.counts;
     out->on_device = 1;
     return DAKC_OK;
   }
@@ -552,8 +697,8 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
     return DAKC_E_NOMEM;
   }
   if (d) {
-    if (hipMemcpy(out->keys, tkeys, kbytes, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(out->counts, ws.counts.p, (size_t)d * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(out->keys, loc.keys, kbytes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(out->counts, loc.counts, (size_t)d * 4, hipMemcpyDeviceToHost) != hipSuccess) {
       dakc_table_free(out);
       return DAKC_E_DEVICE;
     }

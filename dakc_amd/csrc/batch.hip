@@ -1,0 +1,421 @@
+// batch.hip -- batched counting (one multi-job launch per stage, optionally
+// replayed from a captured hipGraph) and the autotuner of the MSD geometry.
+//
+// Many read sets of uneven size are laid out in one arena and counted by the
+// work-queue kernels of the MSD path: every stage numbers its work items across
+// all jobs, so a few hundred small jobs fill the GPU like one large job and
+// cost one launch per stage instead of a few per job.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dk_api_internal.hpp"
+
+namespace dk {
+
+int size_class(uint64_t nmax) {
+  int c = 0;
+  while (c < 63 && (nmax >> (c + 1)) != 0) ++c;
+  return c / 2;  // one class per factor of 4 in k-mer slots
+}
+
+void apply_tuning(const dakc_ctx *c, Plan *pl, int algo) {
+  TuneEntry e;
+  {
+    std::lock_guard<std::mutex> g(const_cast<dakc_ctx *>(c)->tune_mu);
+    auto it = c->tuning.find({pl->kw, size_class(pl->nmax)});
+    if (it != c->tuning.end()) e = it->second;
+  }
+  pl->tune_fin_nt = e.fin_nt;
+  pl->tune_fill_pct = e.fill_pct;
+  plan_choose_path(pl, algo == DAKC_ALGO_AUTO && e.algo == 1 ? DAKC_ALGO_LSD : algo);
+}
+
+size_t msd_job_bytes(const Plan &pl, bool copy_reads);
+
+static size_t hbm_budget(const dakc_ctx *c) {
+  if (c->opts.hbm_budget) return c->opts.hbm_budget;
+  size_t free_b = 0, total_b = 0;
+  hipMemGetInfo(&free_b, &total_b);
+  return (size_t)(free_b * 0.6);
+}
+
+// copy job j's table out of the arena into library-owned host memory
+static int table_to_host(const MsdBatch &b, size_t j, uint64_t n, uint64_t d, dakc_table *out) {
+  const Plan &p = b.plans[j];
+  const TableLoc &L = b.locs[j];
+  std::memset(out, 0, sizeof(*out));
+  out->n = d;
+  out->total = n;
+  out->key_words = p.kw;
+  out->k = p.k;
+  out->num_bins = p.owner_digit ? p.num_pes : 0;
+  const size_t kb = (size_t)d * p.kw * 8;
+  out->keys = (uint64_t *)std::malloc(kb ? kb : 8);
+  out->counts = (uint32_t *)std::malloc(d ? d * 4 : 4);
+  if (p.owner_digit) out->bin_offsets = (uint64_t *)std::calloc((size_t)p.num_pes + 1, 8);
+  if (!out->keys || !out->counts || (p.owner_digit && !out->bin_offsets)) {
+    dakc_table_free(out);
+    return DAKC_E_NOMEM;
+  }
+  if (d && (hipMemcpy(out->keys, L.keys, kb, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(out->counts, L.counts, d * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            (p.owner_digit && hipMemcpy(out->bin_offsets, L.binoff, ((size_t)p.num_pes + 1) * 8,
+                                        hipMemcpyDeviceToHost) != hipSuccess))) {
+    dakc_table_free(out);
+    return DAKC_E_DEVICE;
+  }
+  return DAKC_OK;
+}
+
+// Count the jobs listed in `idx` (all MSD-planned with the same key width and
+// final-level size) in one multi-job launch on ws; overflowed jobs go to `redo`.
+static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std::vector<Plan> &plans,
+                         const std::vector<size_t> &idx, dakc_table *outs, std::vector<int> &status,
+                         std::vector<size_t> &redo) {
+  MsdBatch b;
+  std::vector<Plan> ps;
+  std::vector<bool> copy;
+  std::vector<const dakc_job *> js;
+  for (size_t i : idx) {
+    ps.push_back(plans[i]);
+    copy.push_back(!jobs[i].reads_on_device);
+    js.push_back(&jobs[i]);
+  }
+  msd_layout(b, ps, copy);
+  int s = msd_stage(ws, b, js);
+  if (s) return s;
+  msd_enqueue(ws, b, false);
+  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+  std::vector<uint8_t> meta(b.meta_bytes);
+  if (hipMemcpy(meta.data(), ws.meta.p, b.meta_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    return DAKC_E_DEVICE;
+  for (size_t j = 0; j < idx.size(); ++j) {
+    uint64_t n = 0, d = 0;
+    int st = msd_finish(b, j, meta.data(), &n, &d);
+    if (st == STATUS_MSD_OVERFLOW) {
+      redo.push_back(idx[j]);
+      continue;
+    }
+    if (!st) st = table_to_host(b, j, n, d, &outs[idx[j]]);
+    status[idx[j]] = st;
+  }
+  (void)c;
+  return DAKC_OK;
+}
+
+int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *outs) {
+  hipSetDevice(c->device);
+  std::vector<Plan> plans(njobs);
+  std::vector<int> status(njobs, DAKC_OK);
+  std::vector<size_t> plain;  // counted one at a time (LSD path, or no MSD geometry)
+  std::map<std::pair<int, uint32_t>, std::vector<size_t>> groups;
+  for (size_t i = 0; i < njobs; ++i) {
+    std::memset(&outs[i], 0, sizeof(outs[i]));
+    status[i] = make_plan(&jobs[i], &plans[i]);
+    if (status[i]) continue;
+    apply_tuning(c, &plans[i], c->opts.algo);
+    if (plans[i].msd) groups[{plans[i].kw, plans[i].fin_nt}].push_back(i);
+    else plain.push_back(i);
+  }
+  const size_t budget = hbm_budget(c);
+  std::vector<size_t> redo;
+  for (auto &kv : groups) {
+    // chunks whose arena fits the budget, largest jobs first
+    std::vector<size_t> ids = kv.second;
+    std::sort(ids.begin(), ids.end(), [&](size_t a, size_t b) { return plans[a].nmax > plans[b].nmax; });
+    std::vector<size_t> chunk;
+    size_t bytes = 0;
+    for (size_t k = 0; k <= ids.size(); ++k) {
+      const size_t jb = k < ids.size() ? msd_job_bytes(plans[ids[k]], !jobs[ids[k]].reads_on_device) : 0;
+      if (k == ids.size() || (!chunk.empty() && bytes + jb > budget)) {
+        if (!chunk.empty()) {
+          int s = run_msd_group(c, c->ws, jobs, plans, chunk, outs, status, redo);
+          if (s)
+            for (size_t i : chunk) status[i] = s;
+        }
+        chunk.clear();
+        bytes = 0;
+      }
+      if (k < ids.size()) {
+        chunk.push_back(ids[k]);
+        bytes += jb;
+      }
+    }
+  }
+  // jobs without an MSD geometry, and MSD overflows, on the plain path
+  dakc_job lsd_job;
+  for (size_t i : redo) plain.push_back(i);
+  const int saved_algo = c->opts.algo;
+  for (size_t i : plain) {
+    if (std::find(redo.begin(), redo.end(), i) != redo.end()) c->opts.algo = DAKC_ALGO_LSD;
+    lsd_job = jobs[i];
+    status[i] = dakc_count(c, &lsd_job, &outs[i]);
+    c->opts.algo = saved_algo;
+  }
+  int first = DAKC_OK;
+  for (size_t i = 0; i < njobs; ++i)
+    if (status[i] && !first) first = status[i];
+  return first;
+}
+
+}  // namespace dk
+
+using namespace dk;
+
+// ------------------------------------------------------- captured batches --
+
+struct dakc_batch {
+  dakc_ctx *ctx = nullptr;
+  JobWs ws;
+  hipStream_t stream = nullptr;
+  MsdBatch layout;
+  std::vector<dakc_job> jobs;
+  std::vector<Plan> plans;
+  std::vector<size_t> msd_idx, plain_idx;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ran = false;
+};
+
+extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_batch **out) {
+  if (!ctx || !out || (njobs && !jobs)) return DAKC_E_INVAL;
+  *out = nullptr;
+  hipSetDevice(ctx->device);
+  auto *b = new (std::nothrow) dakc_batch();
+  if (!b) return DAKC_E_NOMEM;
+  b->ctx = ctx;
+  b->jobs.assign(jobs, jobs + njobs);
+  b->plans.resize(njobs);
+  int kw = -1;
+  uint32_t fin = 0;
+  for (size_t i = 0; i < njobs; ++i) {
+    int s = make_plan(&jobs[i], &b->plans[i]);
+    if (s) {
+      delete b;
+      return s;
+    }
+    apply_tuning(ctx, &b->plans[i], ctx->opts.algo);
+    const Plan &p = b->plans[i];
+    if (p.msd && (kw < 0 || (p.kw == kw && p.fin_nt == fin))) {
+      kw = p.kw;
+      fin = p.fin_nt;
+      b->msd_idx.push_back(i);
+    } else {
+      b->plain_idx.push_back(i);
+    }
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      b->ws.init(b->stream) || hipEventCreate(&b->e0) != hipSuccess ||
+      hipEventCreate(&b->e1) != hipSuccess) {
+    dakc_batch_destroy(b);
+    return DAKC_E_DEVICE;
+  }
+  std::vector<Plan> ps;
+  std::vector<bool> copy;
+  std::vector<const dakc_job *> js;
+  for (size_t i : b->msd_idx) {
+    ps.push_back(b->plans[i]);
+    copy.push_back(!jobs[i].reads_on_device);
+    js.push_back(&b->jobs[i]);
+  }
+  msd_layout(b->layout, ps, copy);
+  int s = msd_stage(b->ws, b->layout, js);  // host reads are copied once, here
+  if (s) {
+    dakc_batch_destroy(b);
+    return s;
+  }
+  // capture fills + three levels; replays need no host work
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    dakc_batch_destroy(b);
+    return DAKC_E_DEVICE;
+  }
+  msd_enqueue(b->ws, b->layout, false);
+  if (hipStreamEndCapture(b->stream, &b->graph) != hipSuccess ||
+      hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    dakc_batch_destroy(b);
+    return DAKC_E_DEVICE;
+  }
+  *out = b;
+  return DAKC_OK;
+}
+
+extern "C" int dakc_batch_run(dakc_batch *b, double *device_ms) {
+  if (!b) return DAKC_E_INVAL;
+  hipSetDevice(b->ctx->device);
+  if (b->msd_idx.empty()) {
+    if (device_ms) *device_ms = 0;
+    b->ran = true;
+    return DAKC_OK;
+  }
+  if (hipEventRecord(b->e0, b->stream) != hipSuccess ||
+      hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
+      hipEventRecord(b->e1, b->stream) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+    return DAKC_E_DEVICE;
+  float ms = 0;
+  hipEventElapsedTime(&ms, b->e0, b->e1);
+  if (device_ms) *device_ms = ms;
+  b->ran = true;
+  return DAKC_OK;
+}
+
+extern "C" int dakc_batch_tables(dakc_batch *b, dakc_table *outs) {
+  if (!b || !outs) return DAKC_E_INVAL;
+  hipSetDevice(b->ctx->device);
+  const size_t nj = b->jobs.size();
+  for (size_t i = 0; i < nj; ++i) std::memset(&outs[i], 0, sizeof(outs[i]));
+  int first = DAKC_OK;
+  std::vector<size_t> redo;
+  if (!b->msd_idx.empty()) {
+    if (!b->ran) {
+      int s = dakc_batch_run(b, nullptr);
+      if (s) return s;
+    }
+    std::vector<uint8_t> meta(b->layout.meta_bytes);
+    if (hipMemcpy(meta.data(), b->ws.meta.p, meta.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      return DAKC_E_DEVICE;
+    for (size_t j = 0; j < b->msd_idx.size(); ++j) {
+      uint64_t n = 0, d = 0;
+      int st = msd_finish(b->layout, j, meta.data(), &n, &d);
+      if (st == STATUS_MSD_OVERFLOW) {
+        redo.push_back(b->msd_idx[j]);
+        continue;
+      }
+      if (!st) st = table_to_host(b->layout, j, n, d, &outs[b->msd_idx[j]]);
+      if (st && !first) first = st;
+    }
+  }
+  const int saved = b->ctx->opts.algo;
+  for (size_t i : b->plain_idx) {
+    int st = dakc_count(b->ctx, &b->jobs[i], &outs[i]);
+    if (st && !first) first = st;
+  }
+  b->ctx->opts.algo = DAKC_ALGO_LSD;
+  for (size_t i : redo) {
+    int st = dakc_count(b->ctx, &b->jobs[i], &outs[i]);
+    if (st && !first) first = st;
+  }
+  b->ctx->opts.algo = saved;
+  return first;
+}
+
+extern "C" int dakc_batch_destroy(dakc_batch *b) {
+  if (!b) return DAKC_E_INVAL;
+  hipSetDevice(b->ctx->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->e0) hipEventDestroy(b->e0);
+  if (b->e1) hipEventDestroy(b->e1);
+  b->ws.~JobWs();
+  new (&b->ws) JobWs();
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return DAKC_OK;
+}
+
+// -------------------------------------------------------------- autotuner --
+
+// Time the candidate MSD geometries (final-level workgroup of 256 or 512
+// threads, target sub-bucket fill 50/60/70%) and the LSD path on `job`, keep
+// the fastest for the job's (key width, size class).  The count is exact for
+// every candidate; only the time differs.
+extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, double *best_ms) {
+  if (!ctx || !job) return DAKC_E_INVAL;
+  if (trials < 1) trials = 1;
+  Plan base;
+  int s = make_plan(job, &base);
+  if (s) return s;
+  struct Cand {
+    uint32_t fin, fill, algo;
+  };
+  const Cand cands[] = {{256, 50, 0}, {256, 60, 0}, {256, 70, 0}, {512, 50, 0},
+                        {512, 60, 0}, {512, 70, 0}, {512, 60, 1}};
+  TuneEntry best;
+  best.ms = 1e300;
+  dakc_table t;
+  for (const Cand &cd : cands) {
+    {
+      std::lock_guard<std::mutex> g(ctx->tune_mu);
+      TuneEntry e;
+      e.fin_nt = cd.fin;
+      e.fill_pct = cd.fill;
+      e.algo = cd.algo;
+      ctx->tuning[{base.kw, size_class(base.nmax)}] = e;
+    }
+    double tmin = 1e300;
+    bool ok = true;
+    for (int r = 0; r < trials + 1 && ok; ++r) {  // first run warms the workspace
+      s = dakc_count_device(ctx, job, &t);
+      dakc_table_free(&t);
+      if (s) {
+        ok = false;
+        break;
+      }
+      dakc_stats st;
+      dakc_last_stats(ctx, &st);
+      if (r > 0) tmin = std::min(tmin, st.total_ms + (st.msd_reruns ? 1e9 : 0));
+    }
+    if (ok && tmin < best.ms) {
+      best.fin_nt = cd.fin;
+      best.fill_pct = cd.fill;
+      best.algo = cd.algo;
+      best.ms = tmin;
+    }
+  }
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  ctx->tuning[{base.kw, size_class(base.nmax)}] = best;
+  if (best_ms) *best_ms = best.ms;
+  return DAKC_OK;
+}
+
+extern "C" int dakc_tuning_save(dakc_ctx *ctx, const char *path) {
+  if (!ctx || !path) return DAKC_E_INVAL;
+  std::FILE *f = std::fopen(path, "w");
+  if (!f) return DAKC_E_INVAL;
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  std::fprintf(f, "{\n \"entries\": [\n");
+  size_t i = 0;
+  for (auto &kv : ctx->tuning) {
+    std::fprintf(f, "  {\"key_words\": %d, \"size_class\": %d, \"fin_nt\": %u, \"fill_pct\": %u, "
+                 "\"algo\": \"%s\", \"ms\": %.4f}%s\n",
+                 kv.first.first, kv.first.second, kv.second.fin_nt, kv.second.fill_pct,
+                 kv.second.algo ? "lsd" : "msd", kv.second.ms, ++i < ctx->tuning.size() ? "," : "");
+  }
+  std::fprintf(f, " ]\n}\n");
+  std::fclose(f);
+  return DAKC_OK;
+}
+
+extern "C" int dakc_tuning_load(dakc_ctx *ctx, const char *path) {
+  if (!ctx || !path) return DAKC_E_INVAL;
+  std::FILE *f = std::fopen(path, "r");
+  if (!f) return DAKC_E_INVAL;
+  char line[512];
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  while (std::fgets(line, sizeof line, f)) {
+    int kw, cls;
+    unsigned fin, fill;
+    char algo[8] = {0};
+    double ms;
+    if (std::sscanf(line, " {\"key_words\": %d, \"size_class\": %d, \"fin_nt\": %u, \"fill_pct\": %u, "
+                          "\"algo\": \"%3[a-z]\", \"ms\": %lf}",
+                    &kw, &cls, &fin, &fill, algo, &ms) == 6) {
+      TuneEntry e;
+      e.fin_nt = fin == 256 ? 256 : 512;
+      e.fill_pct = std::min(90u, std::max(30u, fill));
+      e.algo = std::strcmp(algo, "lsd") == 0 ? 1 : 0;
+      e.ms = ms;
+      ctx->tuning[{kw, cls}] = e;
+    }
+  }
+  std::fclose(f);
+  return DAKC_OK;
+}

@@ -39,6 +39,8 @@ struct Plan {
   int msd = 0;
   uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1;
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
+  // tunables set before plan_choose_path (autotuner; defaults otherwise)
+  uint32_t tune_fin_nt = 512, tune_fill_pct = 60;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
@@ -47,10 +49,35 @@ constexpr int STATUS_MSD_OVERFLOW = -100;
 // path choice (dakc_opts.algo)
 int plan_choose_path(Plan *pl, int algo);
 
+// Where a finished count left its table and status words (device pointers).
+struct TableLoc {
+  void *keys = nullptr;
+  uint32_t *counts = nullptr;
+  uint64_t *binoff = nullptr;
+  uint64_t *nd = nullptr;  // [0] k-mers, [1] distinct
+  uint32_t *err = nullptr;
+};
+
+// A laid-out multi-job MSD launch: arena offsets per job and the device
+// argument block (JobDesc table, work-item prefixes, ticket).
+struct MsdBatch {
+  int kw = 1;
+  uint32_t fin_nt = 256;
+  std::vector<Plan> plans;
+  std::vector<size_t> off_reads, off_out1, off_out2, off_counts, off_off1, off_off2, off_flags,
+      off_binoff;
+  std::vector<uint32_t> pre1, pre2, pre3;
+  size_t arena_bytes = 0, flags_off = 0, flags_words = 0;
+  size_t meta_bytes = 0, desc_bytes = 0;
+  std::vector<TableLoc> locs;
+};
+
 // Device workspace of one in-flight job (one per served slot).
 struct JobWs {
   DevBuf hist, dstart, ndev, err, tickets, flags, rle_flags, keys_a, keys_b, counts, binoff, reads,
-      msd_off1, msd_off2;
+      msd_off1, msd_off2, arena, meta, desc;
+  TableLoc loc;     // table of the last enqueued count
+  MsdBatch single;  // layout of the last single-job MSD count
   hipStream_t stream = nullptr;
   hipEvent_t ev[MAX_PASSES + 4] = {};
   uint32_t epoch = 0;
@@ -62,13 +89,29 @@ struct JobWs {
 };
 
 int make_plan(const dakc_job *job, Plan *pl);
+// multi-job MSD: lay out, stage (allocate, upload arguments, copy host reads),
+// enqueue (fills + kernels; capturable in a hipGraph), collect per-job status
+void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads);
+int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs);
+void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events);
+int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, uint64_t *d);
 size_t job_device_bytes(const Plan &pl);
-int enqueue_count(JobWs &ws, const Plan &pl, const uint8_t *dreads, int num_cus, bool events,
-                  void **table_keys);
+int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t *dreads,
+                  int num_cus, bool events);
 int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d);
 
 struct ServeState;  // served mode (serve.hip)
 
+}  // namespace dk
+
+namespace dk {
+// autotuned MSD parameters for one (key width, job-size class)
+struct TuneEntry {
+  uint32_t fin_nt = 512, fill_pct = 60, algo = 0;  // algo: 0 MSD, 1 LSD
+  double ms = 0;
+};
+int size_class(uint64_t nmax);
+void apply_tuning(const dakc_ctx *c, Plan *pl, int algo);
 }  // namespace dk
 
 struct dakc_ctx {
@@ -80,6 +123,8 @@ struct dakc_ctx {
   dakc_stats last = {};
   dk::ServeState *serve = nullptr;
   std::mutex serve_init_mu;
+  std::mutex tune_mu;
+  std::map<std::pair<int, int>, dk::TuneEntry> tuning;  // (kw, size class) -> parameters
 };
 
 namespace dk {

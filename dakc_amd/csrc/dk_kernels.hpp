@@ -52,6 +52,7 @@ struct Workspace {
   void *keys_a, *keys_b;  // nmax keys each (8 or 16 B per key)
   uint32_t *counts;       // nmax
   uint64_t *bin_offsets;  // P + 1
+  uint64_t key_cap;       // keys each of keys_a / keys_b holds
 };
 
 constexpr int MAX_PASSES = 18;  // 16 digits of a 128-bit key + owner digit + spare
@@ -118,15 +119,34 @@ struct MsdArgs {
   uint16_t *off1;    // [nb1 + 1][ntiles1]
   uint16_t *off2;    // [nb1][nd2 + 1][ngroups]
 };
-void launch_l1(int kw, const ReadsDesc &rd, const MsdArgs &m, void *out1, const Workspace &ws,
-               hipStream_t st);
-void launch_l2(int kw, const void *in1, void *out2, const MsdArgs &m, const Workspace &ws,
-               hipStream_t st);
-void launch_fin(int kw, const void *in2, void *out_keys, const MsdArgs &m, const Workspace &ws,
-                uint32_t epoch, hipStream_t st);
+
+// One job of a multi-job MSD launch.  Work items of every stage are numbered
+// across the jobs of a launch (job j owns items [pre[j], pre[j+1])).
+struct JobDesc {
+  ReadsDesc rd;       // records; RT / wpr are the L1 tile geometry
+  MsdArgs m;
+  void *out1;         // L1 tiles (tile * tcap1 keys each); later the table keys
+  void *out2;         // L2 regions ((b * ngroups + g) * cap2 keys each)
+  uint32_t *tcounts;  // table counts
+  uint64_t *nd;       // [0] k-mers, [1] distinct
+  uint64_t *flags;    // nsb look-back words, zeroed before the launch
+  uint32_t *err;      // this job's error bits
+  uint32_t pad;
+};
+struct BatchArgs {
+  const JobDesc *jobs;
+  const uint32_t *pre1, *pre2, *pre3;  // njobs + 1 prefixes of L1 / L2 / final items
+  uint32_t njobs;
+  uint32_t *ticket;                    // final-level ticket
+};
+// launch all three MSD levels for every job of `a` (kw and fin_nt shared by the jobs)
+void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, uint32_t n1, uint32_t n2,
+                      uint32_t n3, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,
                         hipStream_t st);
+void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes,
+                           uint64_t *offsets, hipStream_t st);
 
 }  // namespace dk

@@ -259,6 +259,7 @@ struct SweepArgs {
   uint32_t *ticket;
   DigitSpec dig;
   uint32_t epoch;
+  uint64_t key_cap;  // keys the in / out buffers hold
 };
 
 template <typename KeyT, int NT, int ITEMS, bool FROM_READS>
@@ -290,6 +291,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
     ntiles = (a.rd.nrec + a.rd.RT - 1) / a.rd.RT;
   } else {
     n = a.n_dev[0];
+    if (n > a.key_cap) n = a.key_cap;  // never index past the key buffers
     ntiles = (n + TILE - 1) / TILE;
   }
   if (tile >= ntiles) return;
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
 template <typename KeyT, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint32_t *out_counts,
                                             uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
-                                            uint32_t epoch) {
+                                            uint32_t epoch, uint64_t key_cap) {
   constexpr int TILE = NT * ITEMS;
   __shared__ KeyT s_k[TILE];
   __shared__ uint16_t s_hp[TILE + 1];
@@ -403,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint64_t n = n_dev[0];
+  const uint64_t n = n_dev[0] < key_cap ? n_dev[0] : key_cap;
   const uint64_t ntiles = (n + TILE - 1) / TILE;
   if (tile >= ntiles) return;
   const uint64_t t0 = (uint64_t)tile * TILE;
@@ -608,9 +610,19 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
 // L1: one tile of records -> its k-mers in first-level bucket order, written
 // contiguously at tile * tcap1, with the tile's bucket starts in off1.
 // Also the record-format checks and the k-mer total.
+// job owning work item `item`: largest j < njobs with pre[j] <= item
+DK_DEV uint32_t find_job(const uint32_t *pre, uint32_t njobs, uint32_t item) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_l1(ReadsDesc rd, MsdArgs m, KeyT *out1, uint64_t *n_dev,
-                                           uint32_t *err) {
+__global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
   constexpr int TILE = NT * ITEMS;
   union Smem {
     ExtractSmemBig ex;
@@ -619,8 +631,18 @@ __global__ __launch_bounds__(NT) void k_l1(ReadsDesc rd, MsdArgs m, KeyT *out1, 
   __shared__ Smem sm;
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
+  __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  if (tid == 0) s_job = find_job(a.pre1, a.njobs, item);
+  __syncthreads();
+  const JobDesc &J = a.jobs[s_job];
+  const ReadsDesc rd = J.rd;
+  const MsdArgs m = J.m;
+  KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
+  uint64_t *n_dev = J.nd;
+  uint32_t *err = J.err;
+  const uint32_t tile = item - a.pre1[s_job];
   for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
   KeyT keys[ITEMS];
   uint32_t vmask, dig[ITEMS], rank[ITEMS];
@@ -652,15 +674,24 @@ __global__ __launch_bounds__(NT) void k_l1(ReadsDesc rd, MsdArgs m, KeyT *out1, 
 // digit order, written at (b * ngroups + g) * cap2, with the digit starts in
 // off2[b][digit][g].
 template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_l2(const KeyT *in1, KeyT *out2, MsdArgs m, uint32_t *err) {
+__global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
   constexpr int CAP = NT * ITEMS;
   __shared__ KeyT stage[CAP];
   __shared__ uint32_t s_pref[NT];
   __shared__ uint16_t s_o0[NT];
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
+  __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x;
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  if (tid == 0) s_job = find_job(a.pre2, a.njobs, item);
+  __syncthreads();
+  const JobDesc &J = a.jobs[s_job];
+  const MsdArgs m = J.m;
+  const KeyT *in1 = reinterpret_cast<const KeyT *>(J.out1);
+  KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
+  uint32_t *err = J.err;
+  const uint32_t wg = item - a.pre2[s_job];
   // consecutive workgroups (one XCD's range) take consecutive buckets of the
   // same group: their segments are neighbours in every L1 tile, so the cache
   // lines they share are still in that XCD's L2
@@ -735,9 +766,7 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
 template <typename KeyT, int NT, int ITEMS, int NB3>
-__global__ __launch_bounds__(NT) void k_fin(const KeyT *in2, KeyT *out_keys, uint32_t *out_counts,
-                                            MsdArgs m, uint64_t *n_dev, uint64_t *flags,
-                                            uint32_t *ticket, uint32_t epoch, uint32_t *err) {
+__global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   constexpr int CAP = NT * ITEMS;
   constexpr int TPT = FIN_MAX_GROUPS / NT;
   __shared__ KeyT B[CAP];
@@ -750,16 +779,29 @@ __global__ __launch_bounds__(NT) void k_fin(const KeyT *in2, KeyT *out_keys, uin
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
   __shared__ uint32_t s_mixed[NB3 / 32];
-  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb;
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
-  if (tid == 0) s_sb = atomicAdd(ticket, 1u);
+  if (tid == 0) {
+    const uint32_t item = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
+    s_job = find_job(a.pre3, a.njobs, item);
+    s_sb = item - a.pre3[s_job];
+  }
+  __syncthreads();
+  const JobDesc &J = a.jobs[s_job];
+  const MsdArgs m = J.m;
+  const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
+  KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
+  uint32_t *out_counts = J.tcounts;
+  uint64_t *n_dev = J.nd;
+  uint64_t *flags = J.flags;
+  uint32_t *err = J.err;
+  constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
   const uint32_t nd3 = 1u << m.d3.nbits;
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
   for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
   __syncthreads();
   const uint32_t sb = s_sb, nsb = m.nb1 * m.nd2;
-  if (sb >= nsb) return;
   const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
   const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
 
@@ -910,6 +952,7 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
   a.ticket = ws.tickets + ticket_slot;
   a.dig = dig;
   a.epoch = epoch;
+  a.key_cap = ws.key_cap;
   if (!grid) return;
   if (kw == 1) {
     if (from_reads) k_sweep<uint64_t, SWEEP_NT, SWEEP_ITEMS_1, true><<<grid, SWEEP_NT, 0, st>>>(a);
@@ -927,10 +970,19 @@ void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws,
   if (kw == 1)
     k_rle<uint64_t, SWEEP_NT, SWEEP_ITEMS_1><<<grid, SWEEP_NT, 0, st>>>(
         (const uint64_t *)sorted, (uint64_t *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket,
-        epoch);
+        epoch, ws.key_cap);
   else
     k_rle<K2, SWEEP_NT, SWEEP_ITEMS_2><<<grid, SWEEP_NT, 0, st>>>(
-        (const K2 *)sorted, (K2 *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket, epoch);
+        (const K2 *)sorted, (K2 *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket, epoch,
+        ws.key_cap);
+}
+
+void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes,
+                           uint64_t *offsets, hipStream_t st) {
+  if (kw == 1)
+    k_bin_offsets<uint64_t><<<1, 256, 0, st>>>((const uint64_t *)table_keys, nd, num_pes, offsets);
+  else
+    k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, nd, num_pes, offsets);
 }
 
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,
@@ -942,42 +994,23 @@ void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const 
     k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
 }
 
-void launch_l1(int kw, const ReadsDesc &rd, const MsdArgs &m, void *out1, const Workspace &ws,
-               hipStream_t st) {
-  if (!m.ntiles1) return;
-  if (kw == 1)
-    k_l1<uint64_t, MSD_NT, MSD_ITEMS_1><<<m.ntiles1, MSD_NT, 0, st>>>(rd, m, (uint64_t *)out1, ws.n_dev,
-                                                                       ws.err);
-  else
-    k_l1<K2, MSD_NT, MSD_ITEMS_2><<<m.ntiles1, MSD_NT, 0, st>>>(rd, m, (K2 *)out1, ws.n_dev, ws.err);
-}
-
-void launch_l2(int kw, const void *in1, void *out2, const MsdArgs &m, const Workspace &ws,
-               hipStream_t st) {
-  const uint32_t grid = m.nb1 * m.ngroups;
-  if (!grid) return;
-  if (kw == 1)
-    k_l2<uint64_t, MSD_NT, MSD_ITEMS_1><<<grid, MSD_NT, 0, st>>>((const uint64_t *)in1,
-                                                                  (uint64_t *)out2, m, ws.err);
-  else
-    k_l2<K2, MSD_NT, MSD_ITEMS_2><<<grid, MSD_NT, 0, st>>>((const K2 *)in1, (K2 *)out2, m, ws.err);
-}
-
-void launch_fin(int kw, const void *in2, void *out_keys, const MsdArgs &m, const Workspace &ws,
-                uint32_t epoch, hipStream_t st) {
-  const uint32_t nsb = m.nb1 * m.nd2;
-  uint32_t *ticket = ws.tickets + MAX_PASSES + 1;
-#define DK_FIN(KT, NT_, IT_, NB_)                                                                  \
-  k_fin<KT, NT_, IT_, NB_><<<nsb, NT_, 0, st>>>((const KT *)in2, (KT *)out_keys, ws.counts, m,   \
-                                                ws.n_dev, ws.flags, ticket, epoch, ws.err)
+void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, uint32_t n1, uint32_t n2,
+                      uint32_t n3, hipStream_t st) {
   if (kw == 1) {
-    if (m.fin_nt == 512) DK_FIN(uint64_t, 512, FIN_ITEMS_1, 1024);
-    else DK_FIN(uint64_t, 256, FIN_ITEMS_1, 512);
+    if (n1) k_l1<uint64_t, MSD_NT, MSD_ITEMS_1><<<n1, MSD_NT, 0, st>>>(a);
+    if (n2) k_l2<uint64_t, MSD_NT, MSD_ITEMS_1><<<n2, MSD_NT, 0, st>>>(a);
+    if (n3) {
+      if (fin_nt == 512) k_fin<uint64_t, 512, FIN_ITEMS_1, 1024><<<n3, 512, 0, st>>>(a);
+      else k_fin<uint64_t, 256, FIN_ITEMS_1, 512><<<n3, 256, 0, st>>>(a);
+    }
   } else {
-    if (m.fin_nt == 512) DK_FIN(K2, 512, FIN_ITEMS_2, 1024);
-    else DK_FIN(K2, 256, FIN_ITEMS_2, 512);
+    if (n1) k_l1<K2, MSD_NT, MSD_ITEMS_2><<<n1, MSD_NT, 0, st>>>(a);
+    if (n2) k_l2<K2, MSD_NT, MSD_ITEMS_2><<<n2, MSD_NT, 0, st>>>(a);
+    if (n3) {
+      if (fin_nt == 512) k_fin<K2, 512, FIN_ITEMS_2, 1024><<<n3, 512, 0, st>>>(a);
+      else k_fin<K2, 256, FIN_ITEMS_2, 512><<<n3, 256, 0, st>>>(a);
+    }
   }
-#undef DK_FIN
 }
 
 }  // namespace dk

@@ -30,7 +30,6 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done_ev = nullptr;
   uint64_t ticket = 0;  // 0 = free
-  void *tkeys = nullptr;
 };
 
 struct ServeState {
@@ -52,7 +51,7 @@ static size_t slots_footprint(const ServeState &s) {
     for (const DevBuf *d : {&sl->ws.hist, &sl->ws.dstart, &sl->ws.ndev, &sl->ws.err, &sl->ws.tickets,
                             &sl->ws.flags, &sl->ws.rle_flags, &sl->ws.keys_a, &sl->ws.keys_b,
                             &sl->ws.counts, &sl->ws.binoff, &sl->ws.reads, &sl->ws.msd_off1,
-                            &sl->ws.msd_off2})
+                            &sl->ws.msd_off2, &sl->ws.arena, &sl->ws.meta, &sl->ws.desc})
       b += d->cap;
   return b;
 }
@@ -62,7 +61,7 @@ static size_t slot_growth(const Slot &sl, const Plan &pl, bool need_reads) {
   const size_t need = job_device_bytes(pl) - (need_reads ? 0 : pl.nbytes);
   size_t have = 0;
   for (const DevBuf *d : {&sl.ws.flags, &sl.ws.rle_flags, &sl.ws.keys_a, &sl.ws.keys_b,
-                          &sl.ws.counts, &sl.ws.reads, &sl.ws.msd_off2})
+                          &sl.ws.counts, &sl.ws.reads, &sl.ws.msd_off2, &sl.ws.arena})
     have += d->cap;
   return need > have ? need - have : 0;
 }
@@ -75,14 +74,9 @@ static void complete(ServeState &s, Slot &sl) {
   if (st == STATUS_MSD_OVERFLOW && t.plan.msd) {
     // a sub-bucket did not fit in LDS: count this job again on the LSD path
     plan_choose_path(&t.plan, DAKC_ALGO_LSD);
-    const bool need_reads = !t.job.reads_on_device;
-    const uint8_t *dreads = need_reads ? (const uint8_t *)sl.ws.reads.p : (const uint8_t *)t.job.reads;
-    st = sl.ws.reserve(t.plan, need_reads);
-    if (!st && need_reads && t.plan.nbytes &&
-        hipMemcpyAsync(sl.ws.reads.p, t.job.reads, t.plan.nbytes, hipMemcpyHostToDevice,
-                       sl.stream) != hipSuccess)
-      st = DAKC_E_DEVICE;
-    if (!st) st = enqueue_count(sl.ws, t.plan, dreads, s.ctx->num_cus, false, &sl.tkeys);
+    st = sl.ws.reserve(t.plan, !t.job.reads_on_device);
+    if (!st)
+      st = enqueue_count(sl.ws, t.plan, &t.job, (const uint8_t *)t.job.reads, s.ctx->num_cus, false);
     if (!st && hipStreamSynchronize(sl.stream) != hipSuccess) st = DAKC_E_DEVICE;
     if (!st) st = finish_count(sl.ws, t.plan, &n, &d);
   }
@@ -100,11 +94,11 @@ static void complete(ServeState &s, Slot &sl) {
     if (!o.keys || !o.counts || (t.plan.owner_digit && !o.bin_offsets)) {
       st = DAKC_E_NOMEM;
     } else if (d) {
-      if (hipMemcpyAsync(o.keys, sl.tkeys, kb, hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
-          hipMemcpyAsync(o.counts, sl.ws.counts.p, d * 4, hipMemcpyDeviceToHost, sl.stream) !=
+      if (hipMemcpyAsync(o.keys, sl.ws.loc.keys, kb, hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
+          hipMemcpyAsync(o.counts, sl.ws.loc.counts, d * 4, hipMemcpyDeviceToHost, sl.stream) !=
               hipSuccess ||
           (t.plan.owner_digit &&
-           hipMemcpyAsync(o.bin_offsets, sl.ws.binoff.p, (t.plan.num_pes + 1) * 8,
+           hipMemcpyAsync(o.bin_offsets, sl.ws.loc.binoff, (t.plan.num_pes + 1) * 8,
                           hipMemcpyDeviceToHost, sl.stream) != hipSuccess) ||
           hipStreamSynchronize(sl.stream) != hipSuccess)
         st = DAKC_E_DEVICE;
@@ -145,6 +139,7 @@ static void scheduler(ServeState *s) {
             sl->ws.flags.release();
             sl->ws.rle_flags.release();
             sl->ws.msd_off2.release();
+            sl->ws.arena.release();
           }
         if (slots_footprint(*s) + best > s->budget) break;  // wait for running jobs
       }
@@ -152,14 +147,9 @@ static void scheduler(ServeState *s) {
       s->queue.pop_front();
       Slot &sl = *free_slot;
       int st = sl.ws.reserve(t.plan, need_reads);
-      const uint8_t *dreads = (const uint8_t *)t.job.reads;
-      if (!st && need_reads && t.plan.nbytes) {
-        if (hipMemcpyAsync(sl.ws.reads.p, t.job.reads, t.plan.nbytes, hipMemcpyHostToDevice,
-                           sl.stream) != hipSuccess)
-          st = DAKC_E_DEVICE;
-        dreads = (const uint8_t *)sl.ws.reads.p;
-      }
-      if (!st) st = enqueue_count(sl.ws, t.plan, dreads, s->ctx->num_cus, false, &sl.tkeys);
+      if (!st)
+        st = enqueue_count(sl.ws, t.plan, &t.job, (const uint8_t *)t.job.reads, s->ctx->num_cus,
+                           false);
       if (!st && hipEventRecord(sl.done_ev, sl.stream) != hipSuccess) st = DAKC_E_DEVICE;
       if (st) {
         t.status = st;
@@ -260,7 +250,7 @@ extern "C" int dakc_submit(dakc_ctx *ctx, const dakc_job *job, uint64_t *ticket)
   ServeState *s = serve_get(ctx, &st);
   if (!s) return st;
   if (job_device_bytes(pl) > s->budget) return DAKC_E_REJECTED;
-  plan_choose_path(&pl, ctx->opts.algo);
+  apply_tuning(ctx, &pl, ctx->opts.algo);
   auto t = std::make_unique<Ticket>();
   t->job = *job;
   t->plan = pl;
@@ -301,21 +291,3 @@ extern "C" int dakc_wait(dakc_ctx *ctx, uint64_t ticket, dakc_table *out) {
   s->tickets.erase(it);
   return st;
 }
-
-namespace dk {
-int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *outs) {
-  std::vector<uint64_t> ids(njobs, 0);
-  int first_err = DAKC_OK;
-  for (size_t i = 0; i < njobs; ++i) {
-    std::memset(&outs[i], 0, sizeof(outs[i]));
-    int st = dakc_submit(c, &jobs[i], &ids[i]);
-    if (st && !first_err) first_err = st;
-  }
-  for (size_t i = 0; i < njobs; ++i) {
-    if (!ids[i]) continue;
-    int st = dakc_wait(c, ids[i], &outs[i]);
-    if (st && !first_err) first_err = st;
-  }
-  return first_err;
-}
-}  // namespace dk

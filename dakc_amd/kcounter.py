@@ -158,6 +158,37 @@ class KmerCounter:
             for t in outs:
                 self._libh.dakc_table_free(C.byref(t))
 
+    # captured batches and the autotuner
+    def batch_create(self, read_sets, *, on_device: bool = False, sizes=None, **over) -> "Batch":
+        """Stage many read sets for a captured multi-job launch.  read_sets are
+        host buffers, or device pointers (with `sizes`) when on_device."""
+        if on_device:
+            jobs = [self.job(int(ptr), int(n), on_device=True, **over) for ptr, n in zip(read_sets, sizes)]
+            keep = []
+        else:
+            keep = [_as_bytes(r) for r in read_sets]
+            jobs = [self.job(b.ctypes.data if b.size else 0, int(b.size), **over) for b in keep]
+        arr = (Job * len(jobs))(*jobs)
+        h = C.c_void_p()
+        check(self._libh.dakc_batch_create(self._ctx, arr, len(jobs), C.byref(h)), "dakc_batch_create")
+        return Batch(self, h, len(jobs), keep)
+
+    def autotune(self, reads=None, *, dev_ptr: int = 0, nbytes: int = 0, trials: int = 2, **over) -> float:
+        if reads is not None:
+            buf = _as_bytes(reads)
+            job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+        else:
+            job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        ms = C.c_double()
+        check(self._libh.dakc_autotune(self._ctx, C.byref(job), trials, C.byref(ms)), "dakc_autotune")
+        return ms.value
+
+    def tuning_save(self, path: str) -> None:
+        check(self._libh.dakc_tuning_save(self._ctx, str(path).encode()), "dakc_tuning_save")
+
+    def tuning_load(self, path: str) -> None:
+        check(self._libh.dakc_tuning_load(self._ctx, str(path).encode()), "dakc_tuning_load")
+
     # served mode
     def submit(self, reads, **over) -> tuple[int, np.ndarray]:
         """Queue a job; returns (ticket, buffer) -- keep the buffer alive until wait()."""
@@ -211,6 +242,44 @@ class KmerCounter:
             check(self._libh.dakc_memcpy_h2d(self._ctx, C.c_void_p(dev_ptr),
                                              arr.ctypes.data_as(C.c_void_p), arr.nbytes),
                   "dakc_memcpy_h2d")
+
+
+class Batch:
+    """A captured multi-job count: run() replays the hipGraph, tables() returns
+    one KmerTable per read set."""
+
+    def __init__(self, kc: KmerCounter, handle, njobs: int, keep):
+        self._kc, self._h, self.njobs, self._keep = kc, handle, njobs, keep
+
+    def run(self) -> float:
+        ms = C.c_double()
+        check(self._kc._libh.dakc_batch_run(self._h, C.byref(ms)), "dakc_batch_run")
+        return ms.value
+
+    def tables(self) -> list[KmerTable]:
+        outs = (Table * self.njobs)()
+        st = self._kc._libh.dakc_batch_tables(self._h, outs)
+        try:
+            check(st, "dakc_batch_tables")
+            res = []
+            for t in outs:
+                keys, counts, offs = _lib.table_to_numpy(t)
+                res.append(KmerTable(keys, counts, int(t.k), int(t.total), offs))
+            return res
+        finally:
+            for t in outs:
+                self._kc._libh.dakc_table_free(C.byref(t))
+
+    def close(self) -> None:
+        if self._h:
+            self._kc._libh.dakc_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def gpu_available() -> bool:

@@ -124,8 +124,27 @@ int dakc_count(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
 int dakc_count_device(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
 int dakc_last_stats(dakc_ctx *ctx, dakc_stats *out);
 
-/* Batched: many independent jobs, one table each (outs[i] for jobs[i]). */
+/* Batched: many independent jobs, one table each (outs[i] for jobs[i]).  Jobs
+ * are counted together, one multi-job launch per stage, in chunks that fit the
+ * HBM budget. */
 int dakc_count_batch(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_table *outs);
+
+/* Captured batch: plan and stage the jobs once (host reads are copied to the
+ * device here), capture the whole pipeline in a hipGraph, then replay it with
+ * dakc_batch_run as often as wanted; dakc_batch_tables returns one table per
+ * job.  Device inputs must stay valid until dakc_batch_destroy. */
+typedef struct dakc_batch dakc_batch;
+int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_batch **out);
+int dakc_batch_run(dakc_batch *b, double *device_ms);
+int dakc_batch_tables(dakc_batch *b, dakc_table *outs);
+int dakc_batch_destroy(dakc_batch *b);
+
+/* Autotuner: time the candidate geometries (final-level workgroup size,
+ * sub-bucket fill, MSD or LSD) on a representative job and keep the fastest
+ * for jobs of its key width and size class; save/load the table as JSON. */
+int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, double *best_ms);
+int dakc_tuning_save(dakc_ctx *ctx, const char *path);
+int dakc_tuning_load(dakc_ctx *ctx, const char *path);
 
 /* Served: continuous batching.  submit copies the job descriptor (the reads
  * buffer must stay valid until completion); wait blocks for the table. */

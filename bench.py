@@ -57,7 +57,7 @@ def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
     from dakc_amd import _lib
 
     p = _lib.gen_params(w["seed"], sample_reads, w["read_len"], genome_len=w["genome_len"],
-                        sub_rate=w["sub_rate"])
+                        sub_rate=w["sub_rate"], min_len=w.get("min_len", 0))
     reads = _lib.generate_reads(p, threads=threads)
     oracle.count_mt(reads, w["read_len"], w["k"], threads)  # warm-up
     times = []
@@ -72,12 +72,209 @@ def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
                       f"{t:.3f} s each"}
 
 
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def bench_batch(a) -> int:
+    """Many read sets of uneven size: plain per-set launches vs one captured
+    multi-job launch (autotuned), same device-resident inputs."""
+    from dakc_amd import KmerCounter, _lib
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(5e6), 256)).astype(np.int64)
+    chosen, tot = [], 0
+    for i, nr in enumerate(sizes):
+        if tot + nr > a.batch_reads_cap and chosen:
+            continue
+        chosen.append((i, int(nr)))
+        tot += int(nr)
+    kc = KmerCounter(k=31, read_len=250)
+    ptrs, nbs = [], []
+    for i, nr in chosen:
+        p = _lib.gen_params(7000 + i, nr, 250, min_len=100, sub_rate=0.005)
+        nb = nr * 251
+        d = kc.device_alloc(nb)
+        kc.generate_device(p, d, nb)
+        ptrs.append(d)
+        nbs.append(nb)
+    print(f"[bench] batch: {len(chosen)} read sets, {tot} reads", file=sys.stderr, flush=True)
+    # plain: one count per set, in turn
+    for d, nb in zip(ptrs[:3], nbs[:3]):
+        kc.count_device(d, nb)
+    t0 = time.perf_counter()
+    totals = []
+    for d, nb in zip(ptrs, nbs):
+        t = kc.count_device(d, nb)
+        totals.append((int(t.total), int(t.n)))
+    plain_s = time.perf_counter() - t0
+    kmers = sum(x[0] for x in totals)
+    # autotune one set per size class (a factor of 4 in reads), then one
+    # captured launch for all sets
+    classes = {}
+    for d, nb in zip(ptrs, nbs):
+        classes.setdefault(int(np.log2(nb / 251) // 2), (d, nb))
+    for d, nb in classes.values():
+        kc.autotune(dev_ptr=d, nbytes=nb, trials=1)
+    b = kc.batch_create(ptrs, on_device=True, sizes=nbs)
+    in_launch, held = b.info()
+    for _ in range(a.warmup):
+        b.run()
+    t0 = time.perf_counter()
+    dev_ms = [b.run() for _ in range(a.steps)]
+    batch_s = (time.perf_counter() - t0) / a.steps
+    tabs = b.tables()
+    same = all((t.total, len(t)) == x for t, x in zip(tabs, totals))
+    b.close()
+    for d in ptrs:
+        kc.device_free(d)
+    kc.close()
+    out = {"metric": "k-mers counted/s (batched)", "value": kmers / batch_s, "unit": "k-mers/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": batch_s * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded, generated on the device)",
+           "config": {"workload": f"batched: {len(chosen)} of 256 read sets (sizes log-uniform "
+                                  f"1e4-5e6 reads, taken in seeded order up to {a.batch_reads_cap:g} "
+                                  f"reads), lengths 100-250 M-padded to 250, 0.5% subs, k=31",
+                      "reads": tot, "kmers_per_step": kmers, "parallelism": "single GPU"},
+           "plain_per_job": {"value": kmers / plain_s, "ms": plain_s * 1e3},
+           "batched_over_plain": plain_s / batch_s, "device_ms": float(np.mean(dev_ms)),
+           "sets_in_captured_launch": in_launch, "batch_device_bytes": held,
+           "tables_match_plain": same}
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        w = dict(seed=7000, read_len=250, min_len=100, genome_len=0, sub_rate=0.005, k=31)
+        out["cpu_baseline"] = cpu_baseline(w, 400_000, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_serve(a) -> int:
+    """Poisson-arrival counting requests: continuous batching (dakc_submit /
+    dakc_poll / dakc_wait, jobs overlapping on the device) vs one job at a time
+    in arrival order; latency = arrival -> table ready."""
+    import threading
+    from dakc_amd import KmerCounter, _lib
+    rng = np.random.default_rng(5)
+    nreq = a.serve_requests
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(1e7), nreq)).astype(np.int64)
+    ks = rng.choice([21, 31, 55], nreq)
+    kc = KmerCounter(read_len=150, max_inflight=4)
+    reqs = []
+    for i in range(nreq):
+        p = _lib.gen_params(9000 + i, int(sizes[i]), 150, genome_len=50_000_000, sub_rate=0.005)
+        nb = int(sizes[i]) * 151
+        d = kc.device_alloc(nb)
+        kc.generate_device(p, d, nb)
+        reqs.append((d, nb, int(ks[i])))
+    # service times alone (also the per-request reference totals); both forms
+    # take device-resident reads and leave the table in device memory
+    for d, nb, k in reqs[:4]:
+        kc.count_device(d, nb, k=k)
+    service, totals = [], []
+    for d, nb, k in reqs:
+        t0 = time.perf_counter()
+        t = kc.count_device(d, nb, k=k)
+        service.append(time.perf_counter() - t0)
+        totals.append((int(t.total), int(t.n)))
+    # served warm-up: the first submit allocates the pool
+    for d, nb, k in reqs[:4]:
+        kc.wait_device(kc.submit(None, dev_ptr=d, nbytes=nb, k=k, table_on_device=True)[0]).free()
+    kmers = sum(x[0] for x in totals)
+    print(f"[bench] serve: {nreq} requests, {kmers} k-mers, alone {sum(service):.3f} s",
+          file=sys.stderr, flush=True)
+    lam = 0.75 / float(np.mean(service))  # 75% of one-at-a-time capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    # plain: one at a time, in arrival order
+    start = time.perf_counter()
+    lat_plain = []
+    for (d, nb, k), at in zip(reqs, arrivals):
+        now = time.perf_counter() - start
+        if now < at:
+            time.sleep(at - now)
+        kc.count_device(d, nb, k=k)
+        lat_plain.append(time.perf_counter() - start - at)
+    plain_span = time.perf_counter() - start
+    print(f"[bench] serve: plain leg {plain_span:.3f} s", file=sys.stderr, flush=True)
+    # served: submit on arrival, poll for completion
+    done_t = [None] * nreq
+    tickets = [None] * nreq
+    start = time.perf_counter()
+
+    errors = []
+
+    def submitter():
+        try:
+            for i, ((d, nb, k), at) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - start
+                if now < at:
+                    time.sleep(at - now)
+                tickets[i] = kc.submit(None, dev_ptr=d, nbytes=nb, k=k, table_on_device=True)[0]
+        except Exception as e:  # surfaced by the polling loop below
+            errors.append(e)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    pending = set(range(nreq))
+    same = True
+    while pending:
+        if errors:
+            raise errors[0]
+        for i in list(pending):
+            if tickets[i] is not None and kc.poll(tickets[i]):
+                done_t[i] = time.perf_counter() - start
+                pending.discard(i)
+                t = kc.wait_device(tickets[i])  # done: returns at once
+                same = same and (t.total, t.n) == totals[i]
+                t.free()
+        time.sleep(0.0002)
+    th.join()
+    served_span = max(done_t)
+    print(f"[bench] serve: served leg {served_span:.3f} s", file=sys.stderr, flush=True)
+    lat_served = [done_t[i] - arrivals[i] for i in range(nreq)]
+    # saturated: every request queued at once (closed loop), makespan
+    t0 = time.perf_counter()
+    tks = [kc.submit(None, dev_ptr=d, nbytes=nb, k=k, table_on_device=True)[0] for d, nb, k in reqs]
+    for i, tk in enumerate(tks):
+        t = kc.wait_device(tk)
+        same = same and (t.total, t.n) == totals[i]
+        t.free()
+    sat_served = time.perf_counter() - t0
+    sat_plain = float(np.sum(service))
+    sstats = kc.serve_stats()
+    for d, nb, k in reqs:
+        kc.device_free(d)
+    kc.close()
+    out = {"metric": "served p50/p99 latency and k-mers/s", "value": kmers / served_span,
+           "unit": "k-mers/s", "n_gpus": 1, "steps": nreq, "warmup": 4,
+           "ms_per_step": served_span * 1e3 / nreq, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded, generated on the device)",
+           "config": {"workload": f"served: {nreq} Poisson-arrival requests at 75% of one-at-a-time "
+                                  "capacity, sizes log-uniform 1e4-1e7 reads x 150 bp (genome-like, "
+                                  "0.5% subs), k uniform in {21,31,55}", "kmers": kmers,
+                      "arrival_rate_per_s": lam, "parallelism": "single GPU, 4 job slots"},
+           "served": {"p50_ms": _pct(lat_served, 50) * 1e3, "p99_ms": _pct(lat_served, 99) * 1e3,
+                      "kmers_per_s": kmers / served_span},
+           "plain": {"p50_ms": _pct(lat_plain, 50) * 1e3, "p99_ms": _pct(lat_plain, 99) * 1e3,
+                     "kmers_per_s": kmers / plain_span},
+           "saturated": {"served_kmers_per_s": kmers / sat_served, "plain_kmers_per_s": kmers / sat_plain},
+           "tables_match_alone": same, "serve_stats": sstats}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="genome", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"],
+                    help="plain: one job per count (the headline); batch: many read sets per "
+                         "captured launch vs one launch per set; serve: Poisson-arrival requests, "
+                         "continuous batching vs one job at a time")
+    ap.add_argument("--batch-reads-cap", type=float, default=2e7,
+                    help="batch mode: read sets are taken in seeded order until this many reads")
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -92,6 +289,11 @@ def main() -> int:
         return 2
 
     from dakc_amd import KmerCounter, _lib
+
+    if a.mode == "batch":
+        return bench_batch(a)
+    if a.mode == "serve":
+        return bench_serve(a)
 
     w = WORKLOADS[a.workload]
     kc = KmerCounter(k=w["k"], read_len=w["read_len"],

@@ -29,7 +29,7 @@ EXPORTS = [
     "dakc_count_device", "dakc_last_stats", "dakc_count_batch", "dakc_submit", "dakc_poll",
     "dakc_wait", "dakc_table_free", "dakc_table_serialize", "dakc_gen_nbytes", "dakc_generate_reads",
     "dakc_generate_reads_device", "dakc_device_alloc", "dakc_device_free", "dakc_memcpy_h2d",
-    "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_create", "dakc_batch_run",
+    "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load",
 ]
@@ -48,7 +48,7 @@ class Job(C.Structure):
         ("reads", C.c_void_p), ("nbytes", C.c_uint64), ("format", C.c_int32),
         ("read_len", C.c_int32), ("k", C.c_int32), ("canonical", C.c_int32),
         ("num_pes", C.c_uint32), ("owner_bin", C.c_int32), ("reads_on_device", C.c_int32),
-        ("reserved", C.c_int32),
+        ("table_on_device", C.c_int32),
     ]
 
 
@@ -72,6 +72,16 @@ class Stats(C.Structure):
         ("path", C.c_uint32), ("msd_reruns", C.c_uint32), ("part2_ms", C.c_double),
         ("final_ms", C.c_double),
     ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "admitted", "completed", "failed", "lsd_reruns",
+                                          "budget", "peak_device_bytes", "resident_bytes",
+                                          "pool_used", "queued")] + [("admit_host_ms", C.c_double),
+                                                                     ("init_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class GenParams(C.Structure):
@@ -120,6 +130,8 @@ def load() -> C.CDLL:
         "dakc_memcpy_d2h": (C.c_int, [V, V, V, C.c_uint64]),
         "dakc_memcpy_d2d": (C.c_int, [V, V, V, C.c_uint64]),
         "dakc_batch_create": (C.c_int, [V, P(Job), C.c_size_t, P(V)]),
+        "dakc_batch_info": (C.c_int, [V, P(C.c_size_t), P(C.c_uint64)]),
+        "dakc_serve_get_stats": (C.c_int, [V, P(ServeStats)]),
         "dakc_batch_run": (C.c_int, [V, P(C.c_double)]),
         "dakc_batch_tables": (C.c_int, [V, P(Table)]),
         "dakc_batch_destroy": (C.c_int, [V]),

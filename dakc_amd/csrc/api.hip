@@ -53,16 +53,90 @@ namespace dk {
 
 // ------------------------------------------------------------ workspaces --
 
+DevArena::~DevArena() {
+  if (base) {
+    hipSetDevice(device);
+    hipFree(base);
+  }
+}
+
+int DevArena::init(int dev, size_t bytes) {
+  device = dev;
+  size = bytes & ~(size_t)255;
+  if (hipMalloc(&base, size) != hipSuccess) {
+    (void)hipGetLastError();
+    base = nullptr;
+    size = 0;
+    return DAKC_E_NOMEM;
+  }
+  free_ranges[0] = size;
+  return DAKC_OK;
+}
+
+void *DevArena::alloc(size_t bytes) {
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> g(mu);
+  for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it) {
+    if (it->second < bytes) continue;
+    const size_t off = it->first, len = it->second;
+    free_ranges.erase(it);
+    if (len > bytes) free_ranges[off + bytes] = len - bytes;
+    used += bytes;
+    peak = std::max(peak, used);
+    return (char *)base + off;
+  }
+  return nullptr;
+}
+
+void *DevArena::alloc_top(size_t bytes) {
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> g(mu);
+  for (auto it = free_ranges.rbegin(); it != free_ranges.rend(); ++it) {
+    if (it->second < bytes) continue;
+    const size_t off = it->first, len = it->second;
+    if (len > bytes)
+      it->second = len - bytes;  // the range keeps its bottom part
+    else
+      free_ranges.erase(off);
+    used += bytes;
+    peak = std::max(peak, used);
+    return (char *)base + off + (len - bytes);
+  }
+  return nullptr;
+}
+
+void DevArena::release(void *p, size_t bytes) {
+  if (!p) return;
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> g(mu);
+  size_t off = (size_t)((char *)p - (char *)base), len = bytes;
+  used -= bytes;
+  auto next = free_ranges.lower_bound(off);
+  if (next != free_ranges.end() && off + len == next->first) {
+    len += next->second;
+    next = free_ranges.erase(next);
+  }
+  if (next != free_ranges.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      prev->second += len;
+      return;
+    }
+  }
+  free_ranges[off] = len;
+}
+
 int DevBuf::ensure(size_t bytes, hipStream_t st, bool zero) {
   if (bytes <= cap) return DAKC_OK;
   if (p) {
     hipStreamSynchronize(st);
-    hipFree(p);
-    p = nullptr;
-    cap = 0;
+    release();
   }
   size_t want = (bytes + 4095) & ~(size_t)4095;
-  if (hipMalloc(&p, want) != hipSuccess) {
+  if (arena) {
+    p = arena->alloc(want);
+    if (!p) return DAKC_E_NOMEM;
+  } else if (hipMalloc(&p, want) != hipSuccess) {
     (void)hipGetLastError();
     p = nullptr;
     return DAKC_E_NOMEM;
@@ -73,7 +147,12 @@ int DevBuf::ensure(size_t bytes, hipStream_t st, bool zero) {
 }
 
 void DevBuf::release() {
-  if (p) hipFree(p);
+  if (p) {
+    if (arena)
+      arena->release(p, cap);
+    else
+      hipFree(p);
+  }
   p = nullptr;
   cap = 0;
 }
@@ -84,13 +163,40 @@ JobWs::~JobWs() {
     b->release();
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
+  if (stage_ev) {
+    hipEventSynchronize(stage_ev);
+    hipEventDestroy(stage_ev);
+  }
+  if (hmeta) hipHostFree(hmeta);
+  if (hstage) hipHostFree(hstage);
 }
 
-int JobWs::init(hipStream_t st) {
+int JobWs::init(hipStream_t st, DevArena *pool) {
   stream = st;
+  // the per-job buffers come from the pool when there is one; the small
+  // per-slot buffers stay on hipMalloc
+  for (auto *b : {&flags, &rle_flags, &keys_a, &keys_b, &counts, &reads, &msd_off2, &arena}) b->arena = pool;
   for (auto &e : ev)
     if (hipEventCreate(&e) != hipSuccess) return DAKC_E_DEVICE;
+  if (hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming) != hipSuccess) return DAKC_E_DEVICE;
+  if (hipHostMalloc((void **)&hmeta, 256, hipHostMallocDefault) != hipSuccess) {
+    hmeta = nullptr;
+    return DAKC_E_NOMEM;
+  }
+  std::memset(hmeta, 0, 256);
   return DAKC_OK;
+}
+
+void JobWs::release_job_buffers() {
+  for (auto *b : {&flags, &rle_flags, &keys_a, &keys_b, &counts, &reads, &msd_off2, &arena}) b->release();
+}
+
+void JobWs::detach_job_buffers(std::vector<std::pair<void *, size_t>> &out) {
+  for (auto *b : {&flags, &rle_flags, &keys_a, &keys_b, &counts, &reads, &msd_off2, &arena}) {
+    if (b->p) out.emplace_back(b->p, b->cap);
+    b->p = nullptr;
+    b->cap = 0;
+  }
 }
 
 uint32_t JobWs::next_epoch() {
@@ -405,19 +511,30 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     L.nd = D.nd;
     L.err = D.err;
   }
-  // argument block: JobDesc[nj] | pre1 | pre2 | pre3 | ticket
-  std::vector<uint8_t> host(b.desc_bytes, 0);
-  std::memcpy(host.data(), desc.data(), nj * sizeof(JobDesc));
-  uint32_t *pre = (uint32_t *)(host.data() + al256(nj * sizeof(JobDesc)));
+  // argument block: JobDesc[nj] | pre1 | pre2 | pre3 | ticket, staged in
+  // pinned memory once the previous upload from it has finished
+  hipEventSynchronize(ws.stage_ev);
+  if (ws.hstage_cap < b.desc_bytes) {
+    if (ws.hstage) hipHostFree(ws.hstage);
+    ws.hstage_cap = 0;
+    if (hipHostMalloc((void **)&ws.hstage, b.desc_bytes, hipHostMallocDefault) != hipSuccess) {
+      ws.hstage = nullptr;
+      return DAKC_E_NOMEM;
+    }
+    ws.hstage_cap = b.desc_bytes;
+  }
+  uint8_t *host = ws.hstage;
+  std::memset(host, 0, b.desc_bytes);
+  std::memcpy(host, desc.data(), nj * sizeof(JobDesc));
+  uint32_t *pre = (uint32_t *)(host + al256(nj * sizeof(JobDesc)));
   for (size_t j = 0; j <= nj; ++j) {
     pre[j] = b.pre1[j];
     pre[nj + 1 + j] = b.pre2[j];
     pre[2 * (nj + 1) + j] = b.pre3[j];
   }
-  if (hipMemcpyAsync(dd, host.data(), b.desc_bytes, hipMemcpyHostToDevice, ws.stream) != hipSuccess)
+  if (hipMemcpyAsync(dd, host, b.desc_bytes, hipMemcpyHostToDevice, ws.stream) != hipSuccess ||
+      hipEventRecord(ws.stage_ev, ws.stream) != hipSuccess)
     return DAKC_E_DEVICE;
-  // the staging vector goes out of scope: finish the copy first
-  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
   return DAKC_OK;
 }
 
@@ -478,6 +595,8 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
     if (s) return s;
     msd_enqueue(ws, ws.single, events);
     ws.loc = ws.single.locs[0];
+    if (hipMemcpyAsync(ws.hmeta, ws.loc.nd, META_STRIDE, hipMemcpyDeviceToHost, ws.stream) != hipSuccess)
+      return DAKC_E_DEVICE;
     return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
   }
   hipStream_t st = ws.stream;
@@ -531,21 +650,20 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
   ws.loc.binoff = w.bin_offsets;
   ws.loc.nd = w.n_dev;
   ws.loc.err = w.err;
+  if (hipMemcpyAsync(ws.hmeta, w.n_dev, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(ws.hmeta + 16, w.err, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+    return DAKC_E_DEVICE;
   return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
 }
 
-// After the stream has drained: read n, d and the error word.
+// After the stream has drained: n, d and the error word, from the pinned
+// copy the enqueue made of them.
 int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d) {
-  if (pl.msd) {
-    uint8_t meta[META_STRIDE];
-    if (hipMemcpy(meta, ws.loc.nd, META_STRIDE, hipMemcpyDeviceToHost) != hipSuccess)
-      return DAKC_E_DEVICE;
-    return msd_finish(ws.single, 0, meta, n, d);
-  }
+  if (pl.msd) return msd_finish(ws.single, 0, ws.hmeta, n, d);
   uint64_t nd[2] = {0, 0};
   uint32_t e = 0;
-  if (hipMemcpy(nd, ws.ndev.p, 16, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
-  if (hipMemcpy(&e, ws.err.p, 4, hipMemcpyDeviceToHost) != hipSuccess) return DAKC_E_DEVICE;
+  std::memcpy(nd, ws.hmeta, 16);
+  std::memcpy(&e, ws.hmeta + 16, 4);
   if (e & (ERR_SEPARATOR | ERR_NUL_RECORD)) return DAKC_E_FORMAT;
   if (e & ERR_OVERFLOW) return STATUS_MSD_OVERFLOW;
   if (pl.nmax == 0) nd[1] = 0;
@@ -722,7 +840,9 @@ extern "C" int dakc_last_stats(dakc_ctx *ctx, dakc_stats *out) {
 
 extern "C" void dakc_table_free(dakc_table *t) {
   if (!t) return;
-  if (!t->on_device) {
+  if (t->on_device && t->priv) {
+    dev_table_release(t);
+  } else if (!t->on_device) {
     std::free(t->keys);
     std::free(t->counts);
   }

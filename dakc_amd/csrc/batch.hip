@@ -202,7 +202,15 @@ extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njo
       return s;
     }
     apply_tuning(ctx, &b->plans[i], ctx->opts.algo);
-    const Plan &p = b->plans[i];
+    Plan &p = b->plans[i];
+    // a captured batch is one multi-job MSD launch: take every job it can
+    if (!p.msd && ctx->opts.algo != DAKC_ALGO_LSD) plan_choose_path(&p, DAKC_ALGO_MSD);
+    if (p.msd && p.kw == kw && p.fin_nt != fin) {
+      // one captured launch takes one final-level size: re-plan this job to
+      // the group's (the workgroup size changes the time, never the counts)
+      p.tune_fin_nt = fin;
+      plan_choose_path(&p, DAKC_ALGO_MSD);
+    }
     if (p.msd && (kw < 0 || (p.kw == kw && p.fin_nt == fin))) {
       kw = p.kw;
       fin = p.fin_nt;
@@ -244,6 +252,13 @@ extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njo
     return DAKC_E_DEVICE;
   }
   *out = b;
+  return DAKC_OK;
+}
+
+extern "C" int dakc_batch_info(const dakc_batch *b, size_t *jobs_in_launch, uint64_t *device_bytes) {
+  if (!b) return DAKC_E_INVAL;
+  if (jobs_in_launch) *jobs_in_launch = b->msd_idx.size();
+  if (device_bytes) *device_bytes = b->ws.bytes_held();
   return DAKC_OK;
 }
 

@@ -17,9 +17,32 @@
 namespace dk {
 
 // Grow-only device buffer.
+// Host-side allocator over one device allocation (the served pool): best fit
+// over free ranges, coalesced on release, 256-byte granularity.  Shared by the
+// served scheduler and every device result table cut from it, so the memory
+// lives until the last of them is gone.
+struct DevArena {
+  void *base = nullptr;
+  size_t size = 0;
+  int device = 0;
+  std::mutex mu;
+  std::map<size_t, size_t> free_ranges;  // offset -> length
+  size_t used = 0, peak = 0;
+  ~DevArena();
+  int init(int dev, size_t bytes);
+  // lowest-address first fit (job buffers) / highest-address fit (result
+  // tables), so long-lived tables pack at the top and leave the bottom
+  // contiguous; nullptr when no free range is large enough
+  void *alloc(size_t bytes);
+  void *alloc_top(size_t bytes);
+  void release(void *p, size_t bytes);
+};
+
+// A grow-only device buffer: from hipMalloc, or (served slots) from an arena.
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
+  DevArena *arena = nullptr;
   int ensure(size_t bytes, hipStream_t st, bool zero);
   void release();
 };
@@ -81,11 +104,28 @@ struct JobWs {
   hipStream_t stream = nullptr;
   hipEvent_t ev[MAX_PASSES + 4] = {};
   uint32_t epoch = 0;
+  // pinned host memory: the job's status words land in hmeta (copied at the
+  // end of every enqueue, so finishing a job makes no HIP call), the MSD
+  // argument block is staged in hstage (async upload; stage_ev marks its end)
+  uint8_t *hmeta = nullptr, *hstage = nullptr;
+  size_t hstage_cap = 0;
+  hipEvent_t stage_ev = nullptr;
   ~JobWs();
-  int init(hipStream_t st);
+  int init(hipStream_t st, DevArena *arena = nullptr);
+  void release_job_buffers();  // return the per-job buffers now (no device work may use them)
+  // hand the per-job buffers over as (pointer, bytes) ranges, to be released
+  // once the device work still reading them has finished
+  void detach_job_buffers(std::vector<std::pair<void *, size_t>> &out);
   int reserve(const Plan &pl, bool need_reads);
   uint32_t next_epoch();
   Workspace view() const;
+  size_t bytes_held() const {
+    size_t t = 0;
+    for (const DevBuf *b : {&hist, &dstart, &ndev, &err, &tickets, &flags, &rle_flags, &keys_a, &keys_b,
+                            &counts, &binoff, &reads, &msd_off1, &msd_off2, &arena, &meta, &desc})
+      t += b->cap;
+    return t;
+  }
 };
 
 int make_plan(const dakc_job *job, Plan *pl);
@@ -130,4 +170,16 @@ struct dakc_ctx {
 namespace dk {
 int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *outs);
 void serve_shutdown(dakc_ctx *c);
+
+// Owner of a device-resident result table (dakc_table.priv): the two ranges
+// of the served pool that hold its keys and its counts, kept from the job's
+// own buffers when the rest of them went back to the pool.  `resident` counts
+// the bytes of the live tables of one served context.
+struct DevTableHold {
+  std::shared_ptr<DevArena> arena;
+  std::pair<void *, size_t> parts[2] = {};
+  size_t bytes = 0;
+  std::shared_ptr<std::atomic<uint64_t>> resident;
+};
+void dev_table_release(dakc_table *t);
 }  // namespace dk

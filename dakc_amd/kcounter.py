@@ -111,15 +111,21 @@ class KmerCounter:
 
     def job(self, reads_ptr: int, nbytes: int, *, on_device: bool = False, **over) -> Job:
         p = dict(k=self.k, read_len=self.read_len, canonical=self.canonical,
-                 num_pes=self.num_pes, owner_bin=self.owner_bin)
+                 num_pes=self.num_pes, owner_bin=self.owner_bin, table_on_device=False)
         p.update(over)
         return Job(reads=reads_ptr, nbytes=nbytes, format=_lib.FMT_FIXED, read_len=p["read_len"],
                    k=p["k"], canonical=1 if p["canonical"] else 0, num_pes=p["num_pes"],
-                   owner_bin=p["owner_bin"], reads_on_device=1 if on_device else 0)
+                   owner_bin=p["owner_bin"], reads_on_device=1 if on_device else 0,
+                   table_on_device=1 if p["table_on_device"] else 0)
 
-    def count(self, reads, **over) -> KmerTable:
-        buf = _as_bytes(reads)
-        job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+    def count(self, reads, *, dev_ptr: int = 0, nbytes: int = 0, **over) -> KmerTable:
+        """Count one read buffer into a host table.  With dev_ptr the reads are
+        already in device memory (nbytes long)."""
+        if dev_ptr:
+            job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        else:
+            buf = _as_bytes(reads)
+            job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
         t = Table()
         check(self._libh.dakc_count(self._ctx, C.byref(job), C.byref(t)), "dakc_count")
         try:
@@ -190,22 +196,41 @@ class KmerCounter:
         check(self._libh.dakc_tuning_load(self._ctx, str(path).encode()), "dakc_tuning_load")
 
     # served mode
-    def submit(self, reads, **over) -> tuple[int, np.ndarray]:
-        """Queue a job; returns (ticket, buffer) -- keep the buffer alive until wait()."""
-        buf = _as_bytes(reads)
-        job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+    def submit(self, reads, *, dev_ptr: int = 0, nbytes: int = 0, **over) -> tuple[int, np.ndarray]:
+        """Queue a job; returns (ticket, buffer) -- keep the buffer alive until wait().
+        With dev_ptr the reads are already in device memory (nbytes long)."""
+        if dev_ptr:
+            buf = np.zeros(0, np.uint8)
+            job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        else:
+            buf = _as_bytes(reads)
+            job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
         tk = C.c_uint64()
         check(self._libh.dakc_submit(self._ctx, C.byref(job), C.byref(tk)), "dakc_submit")
         return int(tk.value), buf
+
+    def serve_stats(self) -> dict:
+        st = _lib.ServeStats()
+        check(self._libh.dakc_serve_get_stats(self._ctx, C.byref(st)), "dakc_serve_get_stats")
+        return st.as_dict()
 
     def poll(self, ticket: int) -> bool:
         d = C.c_int()
         check(self._libh.dakc_poll(self._ctx, ticket, C.byref(d)), "dakc_poll")
         return bool(d.value)
 
+    def wait_device(self, ticket: int) -> "DeviceTable":
+        """Result of a job submitted with table_on_device=True: the table stays
+        in device memory until the returned DeviceTable is freed."""
+        t = Table()
+        check(self._libh.dakc_wait(self._ctx, ticket, C.byref(t)), "dakc_wait")
+        return DeviceTable(self, t)
+
     def wait(self, ticket: int) -> KmerTable:
         t = Table()
         check(self._libh.dakc_wait(self._ctx, ticket, C.byref(t)), "dakc_wait")
+        if t.on_device:
+            return DeviceTable(self, t).to_host()
         try:
             keys, counts, offs = _lib.table_to_numpy(t)
             return KmerTable(keys, counts, int(t.k), int(t.total), offs)
@@ -244,12 +269,53 @@ class KmerCounter:
                   "dakc_memcpy_h2d")
 
 
+class DeviceTable:
+    """A served result kept in device memory (keys/counts are device pointers)."""
+
+    def __init__(self, kc: KmerCounter, t: Table):
+        self._kc, self.t = kc, t
+        self.n, self.total, self.key_words = int(t.n), int(t.total), int(t.key_words)
+
+    def to_host(self) -> KmerTable:
+        t = self.t
+        kw, n = self.key_words, self.n
+        keys = self._kc.d2h(_ptr(t.keys), n * kw * 8, np.uint64).reshape(n, kw)
+        counts = self._kc.d2h(_ptr(t.counts), n * 4, np.uint32)
+        offs = None
+        if t.bin_offsets:
+            offs = np.ctypeslib.as_array(t.bin_offsets, shape=(int(t.num_bins) + 1,)).copy()
+        res = KmerTable(keys, counts, int(t.k), self.total, offs)
+        self.free()
+        return res
+
+    def free(self) -> None:
+        if self.t is not None:
+            self._kc._libh.dakc_table_free(C.byref(self.t))
+            self.t = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(p) -> int:
+    return C.cast(p, C.c_void_p).value or 0
+
+
 class Batch:
     """A captured multi-job count: run() replays the hipGraph, tables() returns
     one KmerTable per read set."""
 
     def __init__(self, kc: KmerCounter, handle, njobs: int, keep):
         self._kc, self._h, self.njobs, self._keep = kc, handle, njobs, keep
+
+    def info(self) -> tuple[int, int]:
+        """(jobs counted by the captured launch, device bytes held)."""
+        n, b = C.c_size_t(), C.c_uint64()
+        check(self._kc._libh.dakc_batch_info(self._h, C.byref(n), C.byref(b)), "dakc_batch_info")
+        return int(n.value), int(b.value)
 
     def run(self) -> float:
         ms = C.c_double()

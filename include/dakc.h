@@ -63,14 +63,19 @@ typedef struct dakc_job {
   uint32_t num_pes;    /* P; 1 = single PE */
   int32_t owner_bin;   /* -1: all P owner bins; b >= 0: only k-mers owned by PE b */
   int32_t reads_on_device; /* reads is a device pointer (4-byte aligned) */
-  int32_t reserved;
+  int32_t table_on_device; /* served mode: dakc_wait returns the table in device
+                            * memory owned by the table (keys/counts are device
+                            * pointers; dakc_table_free releases it) */
 } dakc_job;
 
 /* The (kmer, count) table.  keys holds n * key_words u64 (lo, hi per key for
  * key_words == 2); counts holds n u32.  With num_pes > 1 and owner_bin == -1
  * the table is the concatenation of the P owner tables and bin_offsets (P + 1
  * entries) marks their boundaries; otherwise bin_offsets is NULL.
- * Memory is owned by the library: release with dakc_table_free. */
+ * Memory is owned by the library: release with dakc_table_free.  A device
+ * table from dakc_count_device lives in the context's workspace (valid until
+ * the next count on it; dakc_table_free only forgets it); one from a served
+ * job with table_on_device owns its device memory until dakc_table_free. */
 typedef struct dakc_table {
   uint64_t *keys;
   uint32_t *counts;
@@ -135,6 +140,10 @@ int dakc_count_batch(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_tab
  * job.  Device inputs must stay valid until dakc_batch_destroy. */
 typedef struct dakc_batch dakc_batch;
 int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_batch **out);
+/* How many of the batch's jobs the captured launch counts (the rest -- a
+ * second key width, or jobs too large for the multi-job path -- are counted
+ * by dakc_batch_tables one by one) and the device bytes the batch holds. */
+int dakc_batch_info(const dakc_batch *b, size_t *jobs_in_launch, uint64_t *device_bytes);
 int dakc_batch_run(dakc_batch *b, double *device_ms);
 int dakc_batch_tables(dakc_batch *b, dakc_table *outs);
 int dakc_batch_destroy(dakc_batch *b);
@@ -147,10 +156,28 @@ int dakc_tuning_save(dakc_ctx *ctx, const char *path);
 int dakc_tuning_load(dakc_ctx *ctx, const char *path);
 
 /* Served: continuous batching.  submit copies the job descriptor (the reads
- * buffer must stay valid until completion); wait blocks for the table. */
+ * buffer must stay valid until completion); wait blocks for the table.  Jobs
+ * are admitted in FIFO order when their buffers fit the served pool, one
+ * device allocation of opts.hbm_budget bytes (default 60% of free memory at
+ * the first submit) that also holds the device result tables; a job that can
+ * only wait for tables the client already holds fails with DAKC_E_NOMEM. */
 int dakc_submit(dakc_ctx *ctx, const dakc_job *job, uint64_t *ticket);
 int dakc_poll(dakc_ctx *ctx, uint64_t ticket, int *done);
 int dakc_wait(dakc_ctx *ctx, uint64_t ticket, dakc_table *out);
+
+/* Served-mode counters of a context (all zero before the first submit). */
+typedef struct dakc_serve_stats {
+  uint64_t submitted, admitted, completed, failed;
+  uint64_t lsd_reruns;         /* jobs whose MSD count overflowed LDS and reran */
+  uint64_t budget;             /* bytes of the served pool (one device allocation) */
+  uint64_t peak_device_bytes;  /* peak bytes of the pool in use */
+  uint64_t resident_bytes;     /* live device result tables now */
+  uint64_t pool_used;          /* pool bytes in use now (job buffers + tables) */
+  uint64_t queued;             /* jobs waiting for a slot or memory now */
+  double admit_host_ms;        /* host time spent admitting (buffers, enqueue) */
+  double init_ms;              /* time to allocate the pool on the first submit */
+} dakc_serve_stats;
+int dakc_serve_get_stats(dakc_ctx *ctx, dakc_serve_stats *out);
 
 void dakc_table_free(dakc_table *t);
 

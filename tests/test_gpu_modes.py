@@ -125,3 +125,92 @@ def test_autotune_keeps_counts_exact(tmp_path):
         kc2.tuning_load(path)
         t = kc2.count(data)
         assert np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+
+
+def test_captured_batch_takes_every_job_across_tuned_classes(tmp_path):
+    """Size classes tuned to different final-level sizes (and one to the LSD
+    path) still go into the one captured launch, with exact counts."""
+    path = tmp_path / "tune.json"
+    rows = []
+    for cls in range(6, 14):
+        fin, algo = (256, "msd") if cls % 2 else (512, "msd")
+        if cls == 10:
+            algo = "lsd"
+        rows.append(f'  {{"key_words": 1, "size_class": {cls}, "fin_nt": {fin}, "fill_pct": 60, '
+                    f'"algo": "{algo}", "ms": 1.0}}')
+    path.write_text('{\n "entries": [\n' + ",\n".join(rows) + "\n ]\n}\n")
+    sets = read_sets(11, 8)
+    want = [oracle.count(s, 250, 31) for s in sets]
+    with KmerCounter(k=31, read_len=250) as kc:
+        kc.tuning_load(path)
+        with kc.batch_create(sets) as b:
+            in_launch, held = b.info()
+            assert in_launch == len(sets) and held > 0
+            b.run()
+            got = b.tables()
+        for (keys, counts, n, _), t in zip(want, got):
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+
+
+def test_served_device_tables_match_oracle():
+    """Served jobs that keep their table in device memory: each table equals
+    the oracle's, whether read back at once or after later jobs ran."""
+    rng = np.random.default_rng(12)
+    params = [_lib.gen_params(300 + i, int(rng.integers(3000, 40000)), 150, sub_rate=0.005,
+                              genome_len=150_000) for i in range(6)]
+    host = [_lib.generate_reads(p) for p in params]
+    ks = [21, 31, 55, 31, 21, 55]
+    with KmerCounter(read_len=150, max_inflight=3) as kc:
+        tickets = [kc.submit(h, k=k, table_on_device=True)[0] for h, k in zip(host, ks)]
+        held = [kc.wait_device(tk) for tk in tickets]
+        # a few more jobs while the first tables stay resident
+        more = [kc.submit(h, k=k)[0] for h, k in zip(host[:3], ks[:3])]
+        for h, k, tk in zip(host[:3], ks[:3], more):
+            t = kc.wait(tk)
+            keys, counts, n, _ = oracle.count(h, 150, k)
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+        for h, k, dt in zip(host, ks, held):
+            keys, counts, n, _ = oracle.count(h, 150, k)
+            t = dt.to_host()
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+
+
+def test_served_jobs_queue_for_pool_memory():
+    """A pool that holds only a few jobs at a time: later jobs wait for memory
+    (not fail) and every table is exact."""
+    params = [_lib.gen_params(400 + i, 20_000, 150, genome_len=500_000, sub_rate=0.01) for i in range(10)]
+    host = [_lib.generate_reads(p) for p in params]
+    with KmerCounter(k=31, read_len=150, hbm_budget=160 << 20, max_inflight=4) as kc:
+        tickets = [kc.submit(h)[0] for h in host]
+        for h, tk in zip(host, tickets):
+            t = kc.wait(tk)
+            keys, counts, n, _ = oracle.count(h, 150, 31)
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+        st = kc.serve_stats()
+        assert st["completed"] == len(host) and st["failed"] == 0
+        assert st["peak_device_bytes"] <= st["budget"] and st["pool_used"] == 0
+
+
+def test_served_held_device_tables_fill_pool_then_fail_cleanly():
+    """Device tables the client keeps count against the pool: once only they
+    are in the way a new job fails with out-of-memory, and freeing them lets
+    the next job run."""
+    host = _lib.generate_reads(_lib.gen_params(77, 20_000, 150))  # iid: table ~ all k-mers
+    keys, counts, n, _ = oracle.count(host, 150, 31)
+    with KmerCounter(k=31, read_len=150, hbm_budget=128 << 20, max_inflight=2) as kc:
+        held, status = [], None
+        for _ in range(12):
+            tk = kc.submit(host, table_on_device=True)[0]
+            try:
+                held.append(kc.wait_device(tk))
+            except Exception as e:
+                status = getattr(e, "status", None)
+                break
+        assert status == 3 and len(held) >= 1
+        for dt in held:
+            assert dt.total == n and dt.n == len(counts)
+        for dt in held:
+            dt.free()
+        assert kc.serve_stats()["resident_bytes"] == 0
+        t = kc.wait(kc.submit(host)[0])
+        assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)

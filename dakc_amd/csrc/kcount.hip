@@ -762,6 +762,158 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
   return excl;
 }
 
+// Publish sub-bucket sb's distinct count H and return its exclusive prefix
+// over the earlier sub-buckets (decoupled look-back by wavefront 0).  Called
+// by the whole workgroup; ends with a barrier.
+DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t H, uint64_t *n_dev,
+                           uint64_t *s_excl) {
+  constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
+  const uint32_t tid = threadIdx.x;
+  if ((tid >> 6) == 0) {
+    if (tid == 0) lb_store(flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));
+    uint64_t excl = 0;
+    if (sb > 0) {
+      excl = lb_lookback_wave(flags, sb, epoch);
+      if (tid == 0) lb_store(flags + sb, lb_pack(epoch, LB_INC, excl + H));
+    }
+    if (tid == 0) {
+      *s_excl = excl;
+      if (sb == nsb - 1) n_dev[1] = excl + H;
+    }
+  }
+  __syncthreads();
+  return *s_excl;
+}
+
+constexpr uint64_t HASH_EMPTY = ~0ull;  // no k-mer of k < 32 bases has all 64 bits set
+
+// Distinct-key path of the final level (keys of k < 32 bases): the gathered
+// keys go into an LDS hash table (equal keys add to one count), the occupied
+// slots are split by the next b3 bits, the few distinct keys of each bin are
+// ordered by counting, and the (key, count) pairs are written out.  The table
+// is HT slots of key + count; the ordered pairs later take its place.
+// Returns false, having changed nothing the other path needs, when a key
+// finds no slot within 32 probes.  Uniform over the workgroup.
+template <int NT, int ITEMS, int HT>
+DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&keys)[ITEMS],
+                         uint32_t mine, uint64_t *lds, uint32_t *s_h, uint32_t *s_wave,
+                         uint32_t *s_total, uint32_t *s_fail, uint64_t *s_excl, uint32_t sb,
+                         uint32_t nsb) {
+  constexpr int SPT = HT / NT;  // table slots per thread
+  const uint32_t tid = threadIdx.x;
+  uint64_t *tk = lds;                                     // HT keys, then the ordered keys
+  uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);  // HT counts, then the ordered counts
+  uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
+  if (tid == 0) {
+    // once most earlier sub-buckets of the job overflowed the table, skip it
+    const uint32_t f = __hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t o = __hip_atomic_load(hc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_fail = f > o + 32 ? 1u : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    tk[tid + j * NT] = HASH_EMPTY;
+    tc[tid + j * NT] = 0;
+  }
+  __syncthreads();
+  if (*s_fail) return false;
+  // first probes of all keys back to back; collisions walk on from there
+  uint32_t hh[ITEMS];
+  uint64_t old[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      hh[u] = (uint32_t)((keys[u] * 0x9E3779B97F4A7C15ull) >> 40) & (HT - 1);
+      old[u] = atomicCAS((unsigned long long *)&tk[hh[u]], HASH_EMPTY, keys[u]);
+    }
+  }
+  bool lost = false;
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      const uint64_t x = keys[u];
+      uint32_t h = hh[u];
+      uint64_t o = old[u];
+      for (int pr = 0; pr < 32 && o != HASH_EMPTY && o != x; ++pr) {
+        h = (h + 1) & (HT - 1);
+        o = atomicCAS((unsigned long long *)&tk[h], HASH_EMPTY, x);
+      }
+      if (o == HASH_EMPTY || o == x) atomicAdd(&tc[h], 1u);
+      else lost = true;
+    }
+  }
+  if (lost) *s_fail = 1;
+  __syncthreads();
+  const bool fits = !*s_fail;
+  if (tid == 0) atomicAdd(hc + (fits ? 1 : 0), 1u);
+  if (!fits) {
+    __syncthreads();  // everyone has read s_fail before the other path reuses LDS
+    return false;
+  }
+  // split the occupied slots by the next b3 bits
+  const uint32_t nd3 = 1u << m.d3.nbits;
+  uint64_t ek[SPT];
+  uint32_t ec[SPT], ed[SPT], er[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    ek[j] = tk[tid + j * NT];
+    ec[j] = tc[tid + j * NT];
+    if (ek[j] != HASH_EMPTY) {
+      ed[j] = digit_eval(m.d3, ek[j]);
+      er[j] = atomicAdd(&s_h[ed[j]], 1u);
+    }
+  }
+  __syncthreads();
+  scan_lds<NT>(s_h, nd3, s_wave, s_total);
+  const uint32_t H = s_h[nd3];
+  constexpr uint32_t epoch = 1;
+  if (tid == 0) lb_store(J.flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));  // early
+#pragma unroll
+  for (int j = 0; j < SPT; ++j)
+    if (ek[j] != HASH_EMPTY) {
+      tk[s_h[ed[j]] + er[j]] = ek[j];
+      er[j] = s_h[ed[j]];  // bin start
+    }
+  __syncthreads();
+  // order the distinct keys of each bin: place = bin start + number smaller
+  bool big = false;
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    if (ek[j] == HASH_EMPTY) continue;
+    const uint32_t lo = er[j], hi = s_h[ed[j] + 1];
+    uint32_t r = 0;
+    if (hi - lo > 1) {
+      if (hi - lo > 256) {
+        big = true;
+      } else {
+        for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+          uint64_t y[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) y[q] = t0 + q < hi ? tk[t0 + q] : ek[j];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) r += y[q] < ek[j];
+        }
+      }
+    }
+    er[j] = lo + r;
+  }
+  if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SPT; ++j)
+    if (ek[j] != HASH_EMPTY) {
+      tk[er[j]] = ek[j];
+      tc[er[j]] = ec[j];
+    }
+  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, s_excl);
+  uint64_t *out_keys = reinterpret_cast<uint64_t *>(J.out1);
+  for (uint32_t h = tid; h < H; h += NT) {
+    out_keys[excl + h] = tk[h];
+    J.tcounts[excl + h] = tc[h];
+  }
+  return true;
+}
+
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
@@ -779,9 +931,9 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
   __shared__ uint32_t s_mixed[NB3 / 32];
-  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job;
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job, s_fail;
   __shared__ uint64_t s_excl;
-  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t tid = threadIdx.x;
   if (tid == 0) {
     const uint32_t item = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
     s_job = find_job(a.pre3, a.njobs, item);
@@ -794,9 +946,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
   uint32_t *out_counts = J.tcounts;
   uint64_t *n_dev = J.nd;
-  uint64_t *flags = J.flags;
   uint32_t *err = J.err;
-  constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
   const uint32_t nd3 = 1u << m.d3.nbits;
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
   for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
@@ -812,6 +962,14 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
       s_u.seg.pref, s_u.seg.o0, s_wave, &s_total, keys, mine);
   if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
   const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
+
+  if constexpr (sizeof(KeyT) == 8) {
+    static_assert((size_t)NB3 * 4 * 12 <= sizeof(B), "hash table does not fit the key buffer");
+    if (J.rd.k < 32 && fin_distinct<NT, ITEMS, NB3 * 4>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
+                                                       s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
+                                                       nsb))
+      return;
+  }
 
   // split by the next b3 bits into B (unstable, LDS atomics)
   uint32_t dig[ITEMS], rank[ITEMS];
@@ -856,10 +1014,15 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
       const KeyT x = keys[q];
       const uint32_t lo = s_h[dig[q]], hi = s_h[dig[q] + 1];
       if (hi - lo <= RANK_MAX) {
+        // eight independent LDS reads per step; past the bin end y = x, which
+        // counts neither as smaller nor as an earlier copy (t > i there)
         uint32_t r = 0;
-        for (uint32_t t = lo; t < hi; ++t) {
-          const KeyT y = B[t];
-          r += key_lt(y, x) || (t < i && key_eq(y, x));
+        for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+          KeyT y[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) y[j] = t0 + j < hi ? B[t0 + j] : x;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) r += key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x));
         }
         newpos[q] = lo + r;
       } else {
@@ -889,20 +1052,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q)
     if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
-  if (wave == 0) {
-    if (tid == 0) lb_store(flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));
-    uint64_t excl = 0;
-    if (sb > 0) {
-      excl = lb_lookback_wave(flags, sb, epoch);
-      if (tid == 0) lb_store(flags + sb, lb_pack(epoch, LB_INC, excl + H));
-    }
-    if (tid == 0) {
-      s_excl = excl;
-      if (sb == nsb - 1) n_dev[1] = excl + H;
-    }
-  }
-  __syncthreads();
-  const uint64_t excl = s_excl;
+  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, &s_excl);
   for (uint32_t h = tid; h < H; h += NT) {
     const uint32_t i = s_hp[h];
     const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;

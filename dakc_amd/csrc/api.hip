@@ -330,7 +330,7 @@ int plan_choose_path(Plan *pl, int algo) {
   p.msd = 0;
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
-  const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : 512u;
+  const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : p.tune_fin_nt == 1024 ? 1024u : 512u;
   const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap2 = (uint64_t)msd_tile(p.kw);
   const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 60) / 100;
   uint32_t b12 = 0;
@@ -353,7 +353,7 @@ int plan_choose_path(Plan *pl, int algo) {
   if ((uint64_t)rt * p.S > cap2) return 0;
   p.b1 = b1;
   p.b2 = b2;
-  p.b3 = std::min<uint32_t>(fin_nt == 512 ? 10 : 9, bits - b1 - b2);
+  p.b3 = std::min<uint32_t>(fin_nt >= 512 ? 10 : 9, bits - b1 - b2);
   p.fin_nt = fin_nt;
   p.nb1 = owners << b1;
   p.nd2 = 1u << b2;

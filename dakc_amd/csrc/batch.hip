@@ -338,7 +338,7 @@ extern "C" int dakc_batch_destroy(dakc_batch *b) {
 
 // -------------------------------------------------------------- autotuner --
 
-// Time the candidate MSD geometries (final-level workgroup of 256 or 512
+// Time the candidate MSD geometries (final-level workgroup of 256, 512 or 1024
 // threads, target sub-bucket fill 50/60/70%) and the LSD path on `job`, keep
 // the fastest for the job's (key width, size class).  The count is exact for
 // every candidate; only the time differs.
@@ -351,8 +351,8 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
   struct Cand {
     uint32_t fin, fill, algo;
   };
-  const Cand cands[] = {{256, 50, 0}, {256, 60, 0}, {256, 70, 0}, {512, 50, 0},
-                        {512, 60, 0}, {512, 70, 0}, {512, 60, 1}};
+  const Cand cands[] = {{256, 60, 0}, {512, 50, 0}, {512, 60, 0}, {512, 70, 0}, {1024, 50, 0},
+                        {1024, 60, 0}, {1024, 70, 0}, {512, 60, 1}};
   TuneEntry best;
   best.ms = 1e300;
   dakc_table t;
@@ -424,7 +424,7 @@ extern "C" int dakc_tuning_load(dakc_ctx *ctx, const char *path) {
                           "\"algo\": \"%3[a-z]\", \"ms\": %lf}",
                     &kw, &cls, &fin, &fill, algo, &ms) == 6) {
       TuneEntry e;
-      e.fin_nt = fin == 256 ? 256 : 512;
+      e.fin_nt = fin == 256 ? 256 : fin == 1024 ? 1024 : 512;
       e.fill_pct = std::min(90u, std::max(30u, fill));
       e.algo = std::strcmp(algo, "lsd") == 0 ? 1 : 0;
       e.ms = ms;

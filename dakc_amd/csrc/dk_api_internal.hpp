@@ -63,7 +63,7 @@ struct Plan {
   uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1;
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
   // tunables set before plan_choose_path (autotuner; defaults otherwise)
-  uint32_t tune_fin_nt = 512, tune_fill_pct = 60;
+  uint32_t tune_fin_nt = 1024, tune_fill_pct = 60;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
@@ -147,7 +147,7 @@ struct ServeState;  // served mode (serve.hip)
 namespace dk {
 // autotuned MSD parameters for one (key width, job-size class)
 struct TuneEntry {
-  uint32_t fin_nt = 512, fill_pct = 60, algo = 0;  // algo: 0 MSD, 1 LSD
+  uint32_t fin_nt = 1024, fill_pct = 60, algo = 0;  // algo: 0 MSD, 1 LSD
   double ms = 0;
 };
 int size_class(uint64_t nmax);

@@ -70,7 +70,8 @@ inline int tile_keys(int kw) { return SWEEP_NT * (kw == 1 ? SWEEP_ITEMS_1 : SWEE
 // MSD levels 1 and 2: one 1024-thread workgroup per CU handles a tile of
 // 15360 u64 (8192 2x u64) keys in LDS.  Final level: 256-thread workgroups,
 // four per CU, each holding a sub-bucket of up to 3840 u64 (2048 2x u64)
-// keys, or 512-thread workgroups, two per CU, holding twice that.
+// keys, 512-thread workgroups, two per CU, holding twice that, or one
+// 1024-thread workgroup per CU holding four times that.
 constexpr int MSD_NT = 1024;
 constexpr int MSD_ITEMS_1 = 15;
 constexpr int MSD_ITEMS_2 = 8;
@@ -115,7 +116,7 @@ struct MsdArgs {
   uint32_t G;        // L1 tiles per L2 group
   uint32_t ngroups;  // L2 groups per bucket
   uint32_t cap2;     // keys capacity of one (bucket, group)
-  uint32_t fin_nt;   // threads of a final-level workgroup (256 or 512)
+  uint32_t fin_nt;   // threads of a final-level workgroup (256, 512 or 1024)
   uint16_t *off1;    // [nb1 + 1][ntiles1]
   uint16_t *off2;    // [nb1][nd2 + 1][ngroups]
 };

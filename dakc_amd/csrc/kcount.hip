@@ -610,15 +610,16 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
 // L1: one tile of records -> its k-mers in first-level bucket order, written
 // contiguously at tile * tcap1, with the tile's bucket starts in off1.
 // Also the record-format checks and the k-mer total.
-// job owning work item `item`: largest j < njobs with pre[j] <= item
-DK_DEV uint32_t find_job(const uint32_t *pre, uint32_t njobs, uint32_t item) {
-  uint32_t lo = 0, hi = njobs;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (pre[mid] <= item) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+// Job owning work item `item` (items numbered across jobs by the prefix
+// `pre` of njobs + 1 entries), found by the whole workgroup at once: every
+// thread tests its own jobs, one coalesced read instead of a chain of
+// dependent ones.  `item` is uniform; ends with a barrier unless njobs == 1.
+DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, uint32_t *s_job) {
+  if (njobs == 1) return 0;
+  for (uint32_t j = threadIdx.x; j < njobs; j += blockDim.x)
+    if (pre[j] <= item && item < pre[j + 1]) *s_job = j;
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*s_job);
 }
 
 template <typename KeyT, int NT, int ITEMS>
@@ -634,15 +635,14 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
   __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  if (tid == 0) s_job = find_job(a.pre1, a.njobs, item);
-  __syncthreads();
-  const JobDesc &J = a.jobs[s_job];
+  const uint32_t job = find_job_wg(a.pre1, a.njobs, item, &s_job);
+  const JobDesc &J = a.jobs[job];
   const ReadsDesc rd = J.rd;
   const MsdArgs m = J.m;
   KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
   uint64_t *n_dev = J.nd;
   uint32_t *err = J.err;
-  const uint32_t tile = item - a.pre1[s_job];
+  const uint32_t tile = item - a.pre1[job];
   for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
   KeyT keys[ITEMS];
   uint32_t vmask, dig[ITEMS], rank[ITEMS];
@@ -684,14 +684,13 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
   __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  if (tid == 0) s_job = find_job(a.pre2, a.njobs, item);
-  __syncthreads();
-  const JobDesc &J = a.jobs[s_job];
+  const uint32_t job = find_job_wg(a.pre2, a.njobs, item, &s_job);
+  const JobDesc &J = a.jobs[job];
   const MsdArgs m = J.m;
   const KeyT *in1 = reinterpret_cast<const KeyT *>(J.out1);
   KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
   uint32_t *err = J.err;
-  const uint32_t wg = item - a.pre2[s_job];
+  const uint32_t wg = item - a.pre2[job];
   // consecutive workgroups (one XCD's range) take consecutive buckets of the
   // same group: their segments are neighbours in every L1 tile, so the cache
   // lines they share are still in that XCD's L2
@@ -804,12 +803,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   uint64_t *tk = lds;                                     // HT keys, then the ordered keys
   uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);  // HT counts, then the ordered counts
   uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
-  if (tid == 0) {
-    // once most earlier sub-buckets of the job overflowed the table, skip it
-    const uint32_t f = __hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t o = __hip_atomic_load(hc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *s_fail = f > o + 32 ? 1u : 0u;
-  }
+  // *s_fail: the caller's skip decision (set before the keys were loaded)
 #pragma unroll
   for (int j = 0; j < SPT; ++j) {
     tk[tid + j * NT] = HASH_EMPTY;
@@ -917,7 +911,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
-template <typename KeyT, int NT, int ITEMS, int NB3>
+template <typename KeyT, int NT, int ITEMS, int NB3, int HT>
 __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   constexpr int CAP = NT * ITEMS;
   constexpr int TPT = FIN_MAX_GROUPS / NT;
@@ -934,13 +928,19 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job, s_fail;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) {
-    const uint32_t item = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
-    s_job = find_job(a.pre3, a.njobs, item);
-    s_sb = item - a.pre3[s_job];
-  }
+  if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
   __syncthreads();
-  const JobDesc &J = a.jobs[s_job];
+  const uint32_t item = __builtin_amdgcn_readfirstlane(s_sb);
+  const uint32_t job = find_job_wg(a.pre3, a.njobs, item, &s_job);
+  const JobDesc &J = a.jobs[job];
+  if (tid == 0) {
+    // distinct-key path: skip it once most earlier sub-buckets of the job
+    // overflowed the table (read here, before the key loads are in flight)
+    const uint32_t *hc = reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(J.nd) + 24);
+    const uint32_t f = __hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t o = __hip_atomic_load(hc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_fail = f > o + 32 ? 1u : 0u;
+  }
   const MsdArgs m = J.m;
   const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
   KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
@@ -951,7 +951,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
   for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
   __syncthreads();
-  const uint32_t sb = s_sb, nsb = m.nb1 * m.nd2;
+  const uint32_t sb = item - a.pre3[job], nsb = m.nb1 * m.nd2;
   const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
   const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
 
@@ -964,8 +964,8 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
 
   if constexpr (sizeof(KeyT) == 8) {
-    static_assert((size_t)NB3 * 4 * 12 <= sizeof(B), "hash table does not fit the key buffer");
-    if (J.rd.k < 32 && fin_distinct<NT, ITEMS, NB3 * 4>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
+    static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
+    if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
                                                        s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
                                                        nsb))
       return;
@@ -1150,15 +1150,17 @@ void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, uint32_t n1, 
     if (n1) k_l1<uint64_t, MSD_NT, MSD_ITEMS_1><<<n1, MSD_NT, 0, st>>>(a);
     if (n2) k_l2<uint64_t, MSD_NT, MSD_ITEMS_1><<<n2, MSD_NT, 0, st>>>(a);
     if (n3) {
-      if (fin_nt == 512) k_fin<uint64_t, 512, FIN_ITEMS_1, 1024><<<n3, 512, 0, st>>>(a);
-      else k_fin<uint64_t, 256, FIN_ITEMS_1, 512><<<n3, 256, 0, st>>>(a);
+      if (fin_nt == 1024) k_fin<uint64_t, 1024, FIN_ITEMS_1, 1024, 8192><<<n3, 1024, 0, st>>>(a);
+      else if (fin_nt == 512) k_fin<uint64_t, 512, FIN_ITEMS_1, 1024, 4096><<<n3, 512, 0, st>>>(a);
+      else k_fin<uint64_t, 256, FIN_ITEMS_1, 512, 2048><<<n3, 256, 0, st>>>(a);
     }
   } else {
     if (n1) k_l1<K2, MSD_NT, MSD_ITEMS_2><<<n1, MSD_NT, 0, st>>>(a);
     if (n2) k_l2<K2, MSD_NT, MSD_ITEMS_2><<<n2, MSD_NT, 0, st>>>(a);
     if (n3) {
-      if (fin_nt == 512) k_fin<K2, 512, FIN_ITEMS_2, 1024><<<n3, 512, 0, st>>>(a);
-      else k_fin<K2, 256, FIN_ITEMS_2, 512><<<n3, 256, 0, st>>>(a);
+      if (fin_nt == 1024) k_fin<K2, 1024, FIN_ITEMS_2, 1024, 0><<<n3, 1024, 0, st>>>(a);
+      else if (fin_nt == 512) k_fin<K2, 512, FIN_ITEMS_2, 1024, 0><<<n3, 512, 0, st>>>(a);
+      else k_fin<K2, 256, FIN_ITEMS_2, 512, 0><<<n3, 256, 0, st>>>(a);
     }
   }
 }

@@ -35,7 +35,16 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done_ev = nullptr;
   uint64_t ticket = 0;  // 0 = free
+  bool large = false;   // the job fills the GPU by itself
 };
+
+// A job fills the GPU by itself when its first level alone is 16 waves of
+// workgroups over all CUs; two such jobs side by side only share the GPU and
+// stretch each other's latency, so they run one at a time while smaller jobs
+// fill the other slots.
+static bool is_large(const Plan &p, int num_cus) {
+  return p.nmax >= (uint64_t)num_cus * 16 * (uint64_t)msd_tile(p.kw);
+}
 
 struct ServeState {
   dakc_ctx *ctx = nullptr;
@@ -257,8 +266,18 @@ static void scheduler(ServeState *s) {
         ++i;
       }
     }
-    // admit: the first queued job into a free slot if its buffers fit the pool (FIFO)
-    while (!s->queue.empty()) {
+    // admit queued jobs in FIFO order into free slots while their buffers fit
+    // the pool; a large job waits while another large one runs, and smaller
+    // jobs behind it may go first meanwhile (a job that does not fit the
+    // pool keeps its place and everything behind it waits)
+    // under a backlog longer than the slots, throughput comes first: large
+    // jobs may then share the GPU in every slot
+    const int nslots = (int)s->slots.size();
+    const int max_large = (int)s->queue.size() > nslots ? nslots
+                          : s->ctx->opts.max_large > 0 ? s->ctx->opts.max_large : 1;
+    int large_running = 0;
+    for (auto &sl : s->slots) large_running += sl->ticket && sl->large;
+    for (auto qi = s->queue.begin(); qi != s->queue.end();) {
       Slot *slp = nullptr;
       for (auto &sl : s->slots)
         if (!sl->ticket) {
@@ -267,8 +286,13 @@ static void scheduler(ServeState *s) {
         }
       if (!slp) break;
       Slot &sl = *slp;
-      const uint64_t id = s->queue.front();
+      const uint64_t id = *qi;
       Ticket &t = *s->tickets[id];
+      const bool large = is_large(t.plan, s->ctx->num_cus);
+      if (large && large_running >= max_large) {
+        ++qi;
+        continue;
+      }
       const auto h0 = std::chrono::steady_clock::now();
       int st = sl.ws.reserve(t.plan, !t.job.reads_on_device);
       if (!st)
@@ -285,12 +309,12 @@ static void scheduler(ServeState *s) {
           blocked = true;
           break;
         }
-        s->queue.pop_front();
+        qi = s->queue.erase(qi);
         fail(*s, t, DAKC_E_NOMEM);
         progressed = true;
         continue;
       }
-      s->queue.pop_front();
+      qi = s->queue.erase(qi);
       if (st) {
         drop_job_buffers(sl);
         fail(*s, t, st);
@@ -299,6 +323,8 @@ static void scheduler(ServeState *s) {
       }
       ++s->stats.admitted;
       sl.ticket = id;
+      sl.large = large;
+      large_running += large;
       any_running = true;
       progressed = true;
     }

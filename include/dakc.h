@@ -99,6 +99,10 @@ typedef struct dakc_opts {
   uint64_t hbm_budget;  /* served mode: device bytes the job pool may hold (0 = 60% of free) */
   int32_t max_inflight; /* served mode: concurrent jobs on the device (0 = 4) */
   int32_t algo;         /* DAKC_ALGO_* */
+  int32_t max_large;    /* served mode: jobs that fill the GPU by themselves (first level
+                         * >= 16 waves of workgroups) in flight at once while the queue
+                         * is no longer than max_inflight (0 = 1); beyond, any number */
+  int32_t reserved;
 } dakc_opts;
 
 /* Stage timings of the last dakc_count/dakc_count_device on a context,

@@ -132,9 +132,9 @@ def test_captured_batch_takes_every_job_across_tuned_classes(tmp_path):
     path) still go into the one captured launch, with exact counts."""
     path = tmp_path / "tune.json"
     rows = []
-    for cls in range(6, 14):
-        fin, algo = (256, "msd") if cls % 2 else (512, "msd")
-        if cls == 10:
+    for cls in range(12, 28):  # classes: log2 of the k-mer slots
+        fin, algo = (256, "msd") if cls % 2 else (1024, "msd")
+        if cls == 20:
             algo = "lsd"
         rows.append(f'  {{"key_words": 1, "size_class": {cls}, "fin_nt": {fin}, "fill_pct": 60, '
                     f'"algo": "{algo}", "ms": 1.0}}')

@@ -47,7 +47,7 @@ def measured_traffic(workload: str, kernel: str):
     tag = {"genome": "genome_k31", "iid": "iid_k31"}.get(workload)
     data = json.loads(f.read_text()).get(tag, {}).get("kernels", {})
     for name, v in data.items():
-        if name.startswith(f"dk::{kernel}<"):
+        if f"dk::{kernel}<" in name:
             return v["traffic_bytes"]
     return None
 

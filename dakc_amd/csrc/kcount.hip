@@ -102,17 +102,25 @@ DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys
   const uint64_t a0 = b0 & ~3ULL;
   const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
   const uint32_t *src32 = reinterpret_cast<const uint32_t *>(rd.reads + a0);
-  for (uint32_t i = tid; i < nwords; i += NT) {
-    const uint64_t addr = a0 + 4ULL * i;
-    uint32_t w;
-    if (addr + 4 <= rd.nbytes) {
-      w = src32[i];
-    } else {
-      w = 0;
-      for (uint32_t b = 0; b < 4; ++b)
-        if (addr + b < rd.nbytes) w |= (uint32_t)rd.reads[addr + b] << (8 * b);
+  // every word load in flight at once, then the LDS stores
+  constexpr int RW = (SM::RAW_BYTES / 4 + 4 + NT - 1) / NT;
+  uint32_t wv[RW];
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    const uint32_t i = tid + j * NT;
+    wv[j] = 0;
+    if (i < nwords && a0 + 4ULL * i + 4 <= rd.nbytes) wv[j] = src32[i];
+  }
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    const uint32_t i = tid + j * NT;
+    if (i < nwords) {
+      const uint64_t addr = a0 + 4ULL * i;
+      if (addr + 4 > rd.nbytes)
+        for (uint32_t b = 0; b < 4; ++b)
+          if (addr + b < rd.nbytes) wv[j] |= (uint32_t)rd.reads[addr + b] << (8 * b);
+      sm.raw[i] = wv[j];
     }
-    sm.raw[i] = w;
   }
   for (uint32_t r = tid; r < nr; r += NT) sm.mpos[r] = rd.L;
   __syncthreads();
@@ -538,20 +546,34 @@ DK_DEV uint32_t xcd_remap(uint32_t bid, uint32_t n) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
+// LDS scratch of gather_segments for up to NSEG segments and CAP keys.
+template <int NSEG, int CAP>
+struct GatherSmem {
+  static constexpr uint32_t NW = (CAP / 32 + 2) & ~1u;
+  alignas(8) uint32_t wpre[NW];  // segments that start before position 32 * w
+  uint32_t bits[NW];             // bit p: a non-empty segment starts at position p
+  uint32_t delta[NSEG];          // k-th non-empty segment: source offset - first position
+};
+
 // Gather nseg segments into the workgroup's registers.  Segment q holds
-// row1[q] - row0[q] keys at src[region0 + q * stride + row0[q]].  Thread t
-// receives positions [t*per, t*per + mine) of the concatenation, per =
-// ceil(total / NT) <= ITEMS; one binary search per thread finds its first
-// segment and every key load is independent.  Returns the total, or more than
-// cap (and no keys) when the segments do not fit.
-template <typename KeyT, int NT, int ITEMS, int TPT>
+// row1[q] - row0[q] keys at src[region0 + q * stride + row0[q]]; q starts at
+// position pref[q] of the concatenation.  Thread t receives positions
+// u * NT + t (u < ITEMS), so each load instruction of a wavefront reads 64
+// consecutive positions, mostly from one segment.  A position's segment is
+// the number of segment starts at or before it, from a bitmask of the starts
+// and per-word prefix counts (no search).  Returns the total; when it exceeds
+// CAP no key is loaded and mine = 0.  Offsets must fit 32 bits
+// (q * stride + row0[q] < 2^32).
+template <typename KeyT, int NT, int ITEMS, int TPT, int NSEG>
 DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uint16_t *row1,
-                                uint32_t nseg, uint64_t region0, uint64_t stride, uint32_t *s_pref,
-                                uint16_t *s_o0, uint32_t *s_wave, uint32_t *s_total,
-                                KeyT (&v)[ITEMS], uint32_t &mine) {
-  constexpr uint32_t CAP = NT * ITEMS;
-  const uint32_t tid = threadIdx.x;
-  uint32_t o0v[TPT], lenv[TPT], sum = 0;
+                                uint32_t nseg, uint64_t region0, uint32_t stride,
+                                GatherSmem<NSEG, NT * ITEMS> &g, uint32_t *s_wave,
+                                uint32_t *s_total, KeyT (&v)[ITEMS], uint32_t &mine) {
+  constexpr uint32_t CAP = NT * ITEMS, NW = GatherSmem<NSEG, NT * ITEMS>::NW;
+  static_assert(NT * TPT >= NSEG, "segments per thread");
+  static_assert(NW >= NT / 32, "scan scratch");
+  const uint32_t tid = threadIdx.x, lane = __lane_id();
+  uint32_t o0v[TPT], lenv[TPT], sum = 0, cnt = 0;
 #pragma unroll
   for (int u = 0; u < TPT; ++u) {
     const uint32_t q = tid * TPT + u;
@@ -559,51 +581,61 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
     if (q < nseg) {
       o0v[u] = row0[q];
       lenv[u] = row1[q] - o0v[u];
+      if (lenv[u] > CAP + 1) lenv[u] = CAP + 1;  // keeps the sum in 32 bits
     }
     sum += lenv[u];
+    cnt += lenv[u] != 0;
   }
-  uint32_t pre = scan_block<NT>(sum, s_wave, s_total);
+  for (uint32_t w = tid; w < NW; w += NT) g.bits[w] = 0;
+  // exclusive prefix of (keys, non-empty segments) over the threads
+  uint64_t x = ((uint64_t)cnt << 32) | sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  uint64_t *s_w64 = reinterpret_cast<uint64_t *>(g.wpre);  // NT / 64 words, reused below
+  if (lane == 63) s_w64[tid >> 6] = x;
+  __syncthreads();
+  uint64_t pre = 0;
+  for (uint32_t w = 0; w < (tid >> 6); ++w) pre += s_w64[w];
+  if (tid == NT - 1) *s_total = (uint32_t)(pre + x);
+  pre += x - (((uint64_t)cnt << 32) | sum);
+  __syncthreads();
   const uint32_t tot = *s_total;
+  mine = 0;
+  if (tot > CAP) return tot;
+  uint32_t pos = (uint32_t)pre, k = (uint32_t)(pre >> 32);
+  if (tid == 0) g.wpre[0] = 0;
 #pragma unroll
   for (int u = 0; u < TPT; ++u) {
     const uint32_t q = tid * TPT + u;
-    if (q < nseg) {
-      s_pref[q] = pre;
-      s_o0[q] = (uint16_t)o0v[u];
+    if (lenv[u]) {
+      g.delta[k] = q * stride + o0v[u] - pos;
+      atomicOr(&g.bits[pos >> 5], 1u << (pos & 31));
+      // words whose first position lies in (pos, pos + len]: k + 1 starts precede them
+      for (uint32_t w = (pos + 32) >> 5; w <= (pos + lenv[u]) >> 5; ++w) g.wpre[w] = k + 1;
+      pos += lenv[u];
+      ++k;
     }
-    pre += lenv[u];
   }
   __syncthreads();
-  mine = 0;
-  if (tot > CAP) return tot;
-  const uint32_t per = (tot + NT - 1) / NT;
-  const uint32_t p0 = tid * per;
-  if (p0 >= tot) return tot;
-  mine = per < tot - p0 ? per : tot - p0;
-  uint32_t lo = 0, hi = nseg;  // largest q with pref[q] <= p0
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (s_pref[mid] <= p0) lo = mid;
-    else hi = mid;
-  }
-  uint32_t q = lo;
-  uint32_t qnext = q + 1 < nseg ? s_pref[q + 1] : tot;
-  uint64_t addr[ITEMS];
+  mine = tot > tid ? (tot - tid + NT - 1) / NT : 0;
+  if (mine > (uint32_t)ITEMS) mine = ITEMS;
+  uint32_t off[ITEMS];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
-    const uint32_t p = p0 + u;
-    addr[u] = 0;
+    const uint32_t p = u * NT + tid;
+    off[u] = 0;
     if ((uint32_t)u < mine) {
-      while (p >= qnext) {  // zero-length segments share their start
-        ++q;
-        qnext = q + 1 < nseg ? s_pref[q + 1] : tot;
-      }
-      addr[u] = region0 + (uint64_t)q * stride + s_o0[q] + (p - s_pref[q]);
+      const uint32_t w = p >> 5;
+      const uint32_t seg = g.wpre[w] + __popc(g.bits[w] & (0xFFFFFFFFu >> (31 - (p & 31)))) - 1;
+      off[u] = g.delta[seg] + p;
     }
   }
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u)
-    if ((uint32_t)u < mine) v[u] = src[addr[u]];
+    if ((uint32_t)u < mine) v[u] = src[region0 + off[u]];
   return tot;
 }
 
@@ -665,7 +697,13 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
     if ((vmask >> j) & 1) sm.stage[s_h[dig[j]] + rank[j]] = keys[j];
   __syncthreads();
   KeyT *o = out1 + (uint64_t)tile * m.tcap1;
-  for (uint32_t i = tid; i < total; i += NT) o[i] = sm.stage[i];
+  KeyT w[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (tid + u * NT < total) w[u] = sm.stage[tid + u * NT];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (tid + u * NT < total) o[tid + u * NT] = w[u];
   (void)lane;
   (void)wave;
 }
@@ -677,8 +715,7 @@ template <typename KeyT, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
   constexpr int CAP = NT * ITEMS;
   __shared__ KeyT stage[CAP];
-  __shared__ uint32_t s_pref[NT];
-  __shared__ uint16_t s_o0[NT];
+  __shared__ GatherSmem<NT, CAP> s_g;
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
   __shared__ uint32_t s_job;
@@ -701,9 +738,9 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
   for (uint32_t i = tid; i <= m.nd2; i += NT) s_h[i] = 0;
   KeyT v[ITEMS];
   uint32_t mine;
-  uint32_t tot = gather_segments<KeyT, NT, ITEMS, 1>(in1, row0, row0 + m.ntiles1, nt,
-                                                     (uint64_t)t0 * m.tcap1, m.tcap1, s_pref, s_o0,
-                                                     s_wave, &s_total, v, mine);
+  uint32_t tot = gather_segments<KeyT, NT, ITEMS, 1, NT>(in1, row0, row0 + m.ntiles1, nt,
+                                                         (uint64_t)t0 * m.tcap1, m.tcap1, s_g,
+                                                         s_wave, &s_total, v, mine);
   if (tot > (uint32_t)CAP) {
     if (tid == 0) atomicOr(err, ERR_OVERFLOW);
     tot = 0;
@@ -725,7 +762,12 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
     if ((uint32_t)u < mine) stage[s_h[dig[u]] + rank[u]] = v[u];
   __syncthreads();
   KeyT *o = out2 + ((uint64_t)b * m.ngroups + g) * m.cap2;
-  for (uint32_t i = tid; i < tot; i += NT) o[i] = stage[i];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (tid + u * NT < tot) v[u] = stage[tid + u * NT];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (tid + u * NT < tot) o[tid + u * NT] = v[u];
 }
 
 // Exclusive prefix of sub-bucket `idx` over all earlier ones, one wavefront
@@ -917,10 +959,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   constexpr int TPT = FIN_MAX_GROUPS / NT;
   __shared__ KeyT B[CAP];
   __shared__ union {
-    struct {
-      uint32_t pref[FIN_MAX_GROUPS];
-      uint16_t o0[FIN_MAX_GROUPS];
-    } seg;
+    GatherSmem<FIN_MAX_GROUPS, CAP> seg;
     uint16_t hp[CAP];
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
@@ -957,9 +996,9 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
 
   KeyT keys[ITEMS];
   uint32_t mine;
-  const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT>(
-      in2, row0, row0 + m.ngroups, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2,
-      s_u.seg.pref, s_u.seg.o0, s_wave, &s_total, keys, mine);
+  const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
+      in2, row0, row0 + m.ngroups, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg,
+      s_wave, &s_total, keys, mine);
   if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
   const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
 

@@ -324,7 +324,7 @@ int make_plan(const dakc_job *job, Plan *pl) {
 // buckets, fewer key bits when owner bins share them) and b2 <= 10.  L1 tiles
 // take as many records as fill 15360 (8192) slots; an L2 group gathers G
 // consecutive L1 tiles of a bucket, G sized so a group fills 75% of an L2
-// workgroup on average.  b3 (<= 9) splits a sub-bucket into LDS bins.
+// workgroup on average.  b3 (<= 10) splits a sub-bucket into LDS bins.
 int plan_choose_path(Plan *pl, int algo) {
   Plan &p = *pl;
   p.msd = 0;
@@ -523,6 +523,7 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     }
     ws.hstage_cap = b.desc_bytes;
   }
+  if (nj) b.one = desc[0];
   uint8_t *host = ws.hstage;
   std::memset(host, 0, b.desc_bytes);
   std::memcpy(host, desc.data(), nj * sizeof(JobDesc));
@@ -562,11 +563,12 @@ void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
   launch_fill_u64((uint64_t *)ws.arena.p + b.flags_off / 8, 0, b.flags_words, st);
   launch_fill_u32(a.ticket, 0, 1, st);
   if (events) hipEventRecord(ws.ev[0], st);
-  launch_msd_batch(b.kw, b.fin_nt, a, b.pre1[nj], 0, 0, st);
+  const JobDesc *one = nj == 1 ? &b.one : nullptr;
+  launch_msd_batch(b.kw, b.fin_nt, a, one, b.pre1[nj], 0, 0, st);
   if (events) hipEventRecord(ws.ev[1], st);
-  launch_msd_batch(b.kw, b.fin_nt, a, 0, b.pre2[nj], 0, st);
+  launch_msd_batch(b.kw, b.fin_nt, a, one, 0, b.pre2[nj], 0, st);
   if (events) hipEventRecord(ws.ev[2], st);
-  launch_msd_batch(b.kw, b.fin_nt, a, 0, 0, b.pre3[nj], st);
+  launch_msd_batch(b.kw, b.fin_nt, a, one, 0, 0, b.pre3[nj], st);
   for (size_t j = 0; j < nj; ++j)
     if (b.plans[j].owner_digit)
       launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.locs[j].binoff,

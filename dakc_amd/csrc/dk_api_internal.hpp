@@ -93,6 +93,7 @@ struct MsdBatch {
   size_t arena_bytes = 0, flags_off = 0, flags_words = 0;
   size_t meta_bytes = 0, desc_bytes = 0;
   std::vector<TableLoc> locs;
+  JobDesc one;  // the descriptor of job 0, a kernel argument when the batch holds one job
 };
 
 // Device workspace of one in-flight job (one per served slot).

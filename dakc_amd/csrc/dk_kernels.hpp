@@ -141,8 +141,10 @@ struct BatchArgs {
   uint32_t *ticket;                    // final-level ticket
 };
 // launch all three MSD levels for every job of `a` (kw and fin_nt shared by the jobs)
-void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, uint32_t n1, uint32_t n2,
-                      uint32_t n3, hipStream_t st);
+// one: the descriptor of a single-job launch (passed as a kernel argument), or
+// null when the launch holds several jobs (read from a.jobs)
+void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc *one, uint32_t n1,
+                      uint32_t n2, uint32_t n3, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,

@@ -12,6 +12,7 @@
 //   k_bin_offsets owner-bin boundaries when all P owner bins are counted
 #include "dk_device.hpp"
 #include "dk_kernels.hpp"
+#include <cstring>
 
 namespace dk {
 
@@ -622,20 +623,36 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
   __syncthreads();
   mine = tot > tid ? (tot - tid + NT - 1) / NT : 0;
   if (mine > (uint32_t)ITEMS) mine = ITEMS;
-  uint32_t off[ITEMS];
+  // in chunks of CH keys: the word reads, then the segment-offset reads, then
+  // the key loads, so each group of LDS reads waits once and the loads of a
+  // chunk are in flight while the next chunk's offsets are found
+  constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;
 #pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    const uint32_t p = u * NT + tid;
-    off[u] = 0;
-    if ((uint32_t)u < mine) {
-      const uint32_t w = p >> 5;
-      const uint32_t seg = g.wpre[w] + __popc(g.bits[w] & (0xFFFFFFFFu >> (31 - (p & 31)))) - 1;
-      off[u] = g.delta[seg] + p;
+  for (int c0 = 0; c0 < ITEMS; c0 += CH) {
+    uint32_t wp[CH], wb[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int u = c0 + i;
+      const uint32_t w = (u * NT + tid) >> 5;
+      wp[i] = wb[i] = 0;
+      if (u < ITEMS && (uint32_t)u < mine) {
+        wp[i] = g.wpre[w];
+        wb[i] = g.bits[w];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int u = c0 + i;
+      const uint32_t p = u * NT + tid;
+      if (u < ITEMS && (uint32_t)u < mine)
+        wp[i] = g.delta[wp[i] + __popc(wb[i] & (0xFFFFFFFFu >> (31 - (p & 31)))) - 1] + p;
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int u = c0 + i;
+      if (u < ITEMS && (uint32_t)u < mine) v[u] = src[region0 + wp[i]];
     }
   }
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if ((uint32_t)u < mine) v[u] = src[region0 + off[u]];
   return tot;
 }
 
@@ -654,8 +671,8 @@ DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, 
   return __builtin_amdgcn_readfirstlane(*s_job);
 }
 
-template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
+template <typename KeyT, int NT, int ITEMS, bool ONE>
+__global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   constexpr int TILE = NT * ITEMS;
   union Smem {
     ExtractSmemBig ex;
@@ -667,14 +684,14 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
   __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t job = find_job_wg(a.pre1, a.njobs, item, &s_job);
-  const JobDesc &J = a.jobs[job];
+  const uint32_t job = ONE ? 0u : find_job_wg(a.pre1, a.njobs, item, &s_job);
+  const JobDesc &J = ONE ? one : a.jobs[job];
   const ReadsDesc rd = J.rd;
   const MsdArgs m = J.m;
   KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
   uint64_t *n_dev = J.nd;
   uint32_t *err = J.err;
-  const uint32_t tile = item - a.pre1[job];
+  const uint32_t tile = item - (ONE ? 0u : a.pre1[job]);
   for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
   KeyT keys[ITEMS];
   uint32_t vmask, dig[ITEMS], rank[ITEMS];
@@ -711,8 +728,8 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a) {
 // L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
 // digit order, written at (b * ngroups + g) * cap2, with the digit starts in
 // off2[b][digit][g].
-template <typename KeyT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
+template <typename KeyT, int NT, int ITEMS, bool ONE>
+__global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
   constexpr int CAP = NT * ITEMS;
   __shared__ KeyT stage[CAP];
   __shared__ GatherSmem<NT, CAP> s_g;
@@ -721,13 +738,13 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a) {
   __shared__ uint32_t s_job;
   const uint32_t tid = threadIdx.x;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t job = find_job_wg(a.pre2, a.njobs, item, &s_job);
-  const JobDesc &J = a.jobs[job];
+  const uint32_t job = ONE ? 0u : find_job_wg(a.pre2, a.njobs, item, &s_job);
+  const JobDesc &J = ONE ? one : a.jobs[job];
   const MsdArgs m = J.m;
   const KeyT *in1 = reinterpret_cast<const KeyT *>(J.out1);
   KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
   uint32_t *err = J.err;
-  const uint32_t wg = item - a.pre2[job];
+  const uint32_t wg = item - (ONE ? 0u : a.pre2[job]);
   // consecutive workgroups (one XCD's range) take consecutive buckets of the
   // same group: their segments are neighbours in every L1 tile, so the cache
   // lines they share are still in that XCD's L2
@@ -953,8 +970,8 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
-template <typename KeyT, int NT, int ITEMS, int NB3, int HT>
-__global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
+template <typename KeyT, int NT, int ITEMS, int NB3, int HT, bool ONE>
+__global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
   constexpr int CAP = NT * ITEMS;
   constexpr int TPT = FIN_MAX_GROUPS / NT;
   __shared__ KeyT B[CAP];
@@ -970,15 +987,15 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
   __syncthreads();
   const uint32_t item = __builtin_amdgcn_readfirstlane(s_sb);
-  const uint32_t job = find_job_wg(a.pre3, a.njobs, item, &s_job);
-  const JobDesc &J = a.jobs[job];
+  const uint32_t job = ONE ? 0u : find_job_wg(a.pre3, a.njobs, item, &s_job);
+  const JobDesc &J = ONE ? one : a.jobs[job];
   if (tid == 0) {
     // distinct-key path: skip it once most earlier sub-buckets of the job
     // overflowed the table (read here, before the key loads are in flight)
-    const uint32_t *hc = reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(J.nd) + 24);
-    const uint32_t f = __hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t o = __hip_atomic_load(hc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_fail = f > o + 32 ? 1u : 0u;
+    const uint64_t fo = __hip_atomic_load(reinterpret_cast<const uint64_t *>(
+                                              reinterpret_cast<const uint8_t *>(J.nd) + 24),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_fail = (uint32_t)fo > (uint32_t)(fo >> 32) + 32 ? 1u : 0u;
   }
   const MsdArgs m = J.m;
   const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
@@ -990,7 +1007,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a) {
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
   for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
   __syncthreads();
-  const uint32_t sb = item - a.pre3[job], nsb = m.nb1 * m.nd2;
+  const uint32_t sb = item - (ONE ? 0u : a.pre3[job]), nsb = m.nb1 * m.nd2;
   const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
   const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
 
@@ -1183,24 +1200,34 @@ void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const 
     k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
 }
 
-void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, uint32_t n1, uint32_t n2,
-                      uint32_t n3, hipStream_t st) {
+template <typename KeyT, int ITEMS, int NT3, int NB3, int HT, bool ONE>
+static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint32_t n2, uint32_t n3,
+                       hipStream_t st) {
+  if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<n1, MSD_NT, 0, st>>>(a, one);
+  if (n2) k_l2<KeyT, MSD_NT, ITEMS, ONE><<<n2, MSD_NT, 0, st>>>(a, one);
+  constexpr int FI = sizeof(KeyT) == 8 ? FIN_ITEMS_1 : FIN_ITEMS_2;
+  if (n3) k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<n3, NT3, 0, st>>>(a, one);
+}
+
+template <typename KeyT, int ITEMS, bool ONE>
+static void msd_levels_nt(uint32_t fin_nt, const BatchArgs &a, const JobDesc &one, uint32_t n1,
+                          uint32_t n2, uint32_t n3, hipStream_t st) {
+  constexpr bool W1 = sizeof(KeyT) == 8;
+  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 1024, W1 ? 8192 : 0, ONE>(a, one, n1, n2, n3, st);
+  else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 4096 : 0, ONE>(a, one, n1, n2, n3, st);
+  else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2048 : 0, ONE>(a, one, n1, n2, n3, st);
+}
+
+void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc *one, uint32_t n1,
+                      uint32_t n2, uint32_t n3, hipStream_t st) {
+  JobDesc none;
+  std::memset(&none, 0, sizeof(none));
   if (kw == 1) {
-    if (n1) k_l1<uint64_t, MSD_NT, MSD_ITEMS_1><<<n1, MSD_NT, 0, st>>>(a);
-    if (n2) k_l2<uint64_t, MSD_NT, MSD_ITEMS_1><<<n2, MSD_NT, 0, st>>>(a);
-    if (n3) {
-      if (fin_nt == 1024) k_fin<uint64_t, 1024, FIN_ITEMS_1, 1024, 8192><<<n3, 1024, 0, st>>>(a);
-      else if (fin_nt == 512) k_fin<uint64_t, 512, FIN_ITEMS_1, 1024, 4096><<<n3, 512, 0, st>>>(a);
-      else k_fin<uint64_t, 256, FIN_ITEMS_1, 512, 2048><<<n3, 256, 0, st>>>(a);
-    }
+    if (one) msd_levels_nt<uint64_t, MSD_ITEMS_1, true>(fin_nt, a, *one, n1, n2, n3, st);
+    else msd_levels_nt<uint64_t, MSD_ITEMS_1, false>(fin_nt, a, none, n1, n2, n3, st);
   } else {
-    if (n1) k_l1<K2, MSD_NT, MSD_ITEMS_2><<<n1, MSD_NT, 0, st>>>(a);
-    if (n2) k_l2<K2, MSD_NT, MSD_ITEMS_2><<<n2, MSD_NT, 0, st>>>(a);
-    if (n3) {
-      if (fin_nt == 1024) k_fin<K2, 1024, FIN_ITEMS_2, 1024, 0><<<n3, 1024, 0, st>>>(a);
-      else if (fin_nt == 512) k_fin<K2, 512, FIN_ITEMS_2, 1024, 0><<<n3, 512, 0, st>>>(a);
-      else k_fin<K2, 256, FIN_ITEMS_2, 512, 0><<<n3, 256, 0, st>>>(a);
-    }
+    if (one) msd_levels_nt<K2, MSD_ITEMS_2, true>(fin_nt, a, *one, n1, n2, n3, st);
+    else msd_levels_nt<K2, MSD_ITEMS_2, false>(fin_nt, a, none, n1, n2, n3, st);
   }
 }
 

@@ -343,9 +343,10 @@ int plan_choose_path(Plan *pl, int algo) {
   if (b12 > b1 + 10) b1 = std::min(b1max, b12 - 10);
   const uint32_t b2 = std::min(std::min<uint32_t>(10, b12 > b1 ? b12 - b1 : 0), bits - b1);
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
-  // L1 tile geometry
+  // L1 tile geometry: a record's slots take ceil(S / items) threads
   const uint32_t wpr = (p.L + 31) / 32 + 2;
-  uint32_t rt = p.S ? (uint32_t)(cap2 / p.S) : (uint32_t)cap2;
+  const uint32_t l1_items = p.kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2;
+  uint32_t rt = p.S ? (uint32_t)MSD_NT / ((p.S + l1_items - 1) / l1_items) : (uint32_t)cap2;
   rt = std::min<uint32_t>(rt, L1_MAXRT);
   rt = std::min<uint32_t>(rt, L1_RAW / p.stride);
   rt = std::min<uint32_t>(rt, L1_CODEW / wpr);

@@ -12,6 +12,7 @@
 //   k_bin_offsets owner-bin boundaries when all P owner bins are counted
 #include "dk_device.hpp"
 #include "dk_kernels.hpp"
+#include <algorithm>
 #include <cstring>
 
 namespace dk {
@@ -656,9 +657,6 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
   return tot;
 }
 
-// L1: one tile of records -> its k-mers in first-level bucket order, written
-// contiguously at tile * tcap1, with the tile's bucket starts in off1.
-// Also the record-format checks and the k-mer total.
 // Job owning work item `item` (items numbered across jobs by the prefix
 // `pre` of njobs + 1 entries), found by the whole workgroup at once: every
 // thread tests its own jobs, one coalesced read instead of a chain of
@@ -671,9 +669,197 @@ DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, 
   return __builtin_amdgcn_readfirstlane(*s_job);
 }
 
+// ---------------------------------------------------------- L1 extraction --
+//
+// k_l1 tiles are record-aligned: a record's S window slots are split over
+// TPR = ceil(S / ITEMS) consecutive threads of ITEMS consecutive slots each,
+// so a thread's windows slide along one record: after the first, each k-mer
+// is the previous one shifted by one base (kcounter_funcs.cpp:25-38 does the
+// same per read).
+
+// Raw words of a tile's records, loaded into registers (every load in flight
+// at once); word i of the tile starts at byte a0 + 4i, a0 = first record
+// start rounded down to 4.
+// The thread index, opaque to the compiler: per-thread address arithmetic is
+// recomputed in each tile of a persistent loop instead of being hoisted out
+// of it and spilled.
+DK_DEV uint32_t opaque_tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+template <int NT, int RW>
+DK_DEV void l1_load_raw(const ReadsDesc &rd, uint32_t tile, uint32_t (&wv)[RW]) {
+  const uint32_t tid = opaque_tid();
+  const uint64_t r0 = (uint64_t)tile * rd.RT;
+  const uint64_t left = rd.nrec - r0;
+  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
+  const uint64_t b0 = r0 * rd.stride;
+  uint64_t b1 = b0 + (uint64_t)nr * rd.stride;
+  if (b1 > rd.nbytes) b1 = rd.nbytes;
+  const uint64_t a0 = b0 & ~3ULL;
+  const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
+  const uint32_t *src32 = reinterpret_cast<const uint32_t *>(rd.reads + a0);
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    const uint32_t i = tid + j * NT;
+    wv[j] = 0;
+    if (i < nwords && a0 + 4ULL * i + 4 <= rd.nbytes) wv[j] = src32[i];
+  }
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    const uint32_t i = tid + j * NT;
+    const uint64_t addr = a0 + 4ULL * i;
+    if (i < nwords && addr + 4 > rd.nbytes)  // the job's last bytes: not a whole word
+      for (uint32_t b = 0; b < 4; ++b)
+        if (addr + b < rd.nbytes) wv[j] |= (uint32_t)rd.reads[addr + b] << (8 * b);
+  }
+}
+
+// Sliding-window helpers.  key_push appends base b to a forward k-mer;
+// rc_push prepends the complement of b to its reverse complement.
+DK_DEV void key_push(uint64_t &x, uint32_t b, uint64_t kmask) { x = ((x << 2) | b) & kmask; }
+DK_DEV void key_push(K2 &x, uint32_t b, uint64_t kmask) {
+  x.hi = ((x.hi << 2) | (x.lo >> 62)) & kmask;
+  x.lo = (x.lo << 2) | b;
+}
+DK_DEV void rc_push(uint64_t &x, uint32_t b, int k) { x = (x >> 2) | ((uint64_t)(b ^ 3u) << (2 * k - 2)); }
+DK_DEV void rc_push(K2 &x, uint32_t b, int k) {  // 33 <= k <= 64: the new base is in the high word
+  x.lo = (x.lo >> 2) | (x.hi << 62);
+  x.hi = (x.hi >> 2) | ((uint64_t)(b ^ 3u) << (2 * k - 66));
+}
+
+// l1_stage_raw puts the tile held in wv into LDS; l1_windows decodes it
+// (records r0 .. r0 + nr - 1 of the job) into 2-bit codes, invalid-base
+// masks and first-'M' positions, then builds this thread's ITEMS windows.  Restates kcounter_funcs.cpp:40-111: a window
+// is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it starts
+// before the record's first 'M' (0xF0 ends the read).  Ends with the
+// thread's keys; the LDS is still read by other threads.
+template <int NT, int RW, typename SM>
+DK_DEV void l1_stage_raw(const ReadsDesc &rd, uint32_t tile, const uint32_t (&wv)[RW], SM &sm) {
+  const uint32_t tid = opaque_tid();
+  const uint64_t r0 = (uint64_t)tile * rd.RT;
+  const uint64_t left = rd.nrec - r0;
+  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
+  const uint64_t b0 = r0 * rd.stride;
+  uint64_t b1 = b0 + (uint64_t)nr * rd.stride;
+  if (b1 > rd.nbytes) b1 = rd.nbytes;
+  const uint64_t a0 = b0 & ~3ULL;
+  const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    const uint32_t i = tid + j * NT;
+    if (i < nwords) sm.raw[i] = wv[j];
+  }
+  for (uint32_t r = tid; r < nr; r += NT) sm.mpos[r] = rd.L;
+}
+
+template <typename KeyT, int NT, int ITEMS, typename SM>
+DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[ITEMS],
+                       uint32_t &vmask, uint32_t *err) {
+  const uint32_t tid = opaque_tid();
+  const uint64_t r0 = (uint64_t)tile * rd.RT;
+  const uint64_t left = rd.nrec - r0;
+  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
+  const uint64_t b0 = r0 * rd.stride;
+  const uint64_t a0 = b0 & ~3ULL;
+  __syncthreads();  // the raw records are in LDS
+
+  const uint8_t *raw = reinterpret_cast<const uint8_t *>(sm.raw) + (b0 - a0);
+  const uint32_t wpl = (rd.L + 31) >> 5;
+  for (uint32_t it = tid; it < nr * rd.wpr; it += NT) {
+    const uint32_t r = it / rd.wpr, w = it - r * rd.wpr;
+    uint64_t code = 0;
+    uint32_t inv = 0xFFFFFFFFu;
+    if (w < wpl) {
+      const uint8_t *rec = raw + (uint64_t)r * rd.stride;
+      const uint32_t j0 = w * 32;
+      uint32_t mloc = 0xFFFFFFFFu;
+      inv = 0;
+#pragma unroll 8
+      for (uint32_t q = 0; q < 32; ++q) {
+        const uint32_t j = j0 + q;
+        const uint32_t b = j < rd.L ? rec[j] : 0u;
+        const uint32_t u = b & 0xDFu;  // upper-case letters
+        const bool ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T');
+        const uint32_t t = (b >> 1) & 3u;        // A0 C1 T2 G3 (ASCII bits 1-2)
+        const uint32_t c = t ^ ((t >> 1) ^ 1u);  // -> C0 A1 T2 G3
+        code = (code << 2) | (ok ? c : 0u);
+        inv |= (ok ? 0u : 1u) << q;
+        if (u == 'M' && mloc == 0xFFFFFFFFu) mloc = j;
+      }
+      if (mloc != 0xFFFFFFFFu) atomicMin(&sm.mpos[r], mloc);
+      if (w == 0 && rec[0] == 0) atomicOr(err, ERR_NUL_RECORD);
+      if (w == wpl - 1 && b0 + (uint64_t)(r + 1) * rd.stride <= rd.nbytes && rec[rd.L] != '\n')
+        atomicOr(err, ERR_SEPARATOR);
+    }
+    sm.code[it] = code;
+    sm.inv[it] = inv;
+  }
+  __syncthreads();
+
+  vmask = 0;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) keys[j] = KeyT{};
+  if (rd.S == 0) return;  // reads shorter than k: no k-mer slots
+  const int k = (int)rd.k;
+  const uint32_t tpr = (rd.S + ITEMS - 1) / ITEMS;
+  const uint32_t r = tid / tpr, p0 = (tid - r * tpr) * ITEMS;
+  if (r >= nr) return;
+  const uint32_t nslots = rd.S - p0 < (uint32_t)ITEMS ? rd.S - p0 : (uint32_t)ITEMS;
+  const uint32_t rb = r * rd.wpr, w0 = rb + (p0 >> 5), sh0 = p0 & 31;
+  const uint32_t mp = sm.mpos[r];
+  // invalid-base bits from p0 on: vlo = bits [p0, p0 + 64), vhi = the next 32
+  const uint64_t i01 = (uint64_t)sm.inv[w0] | ((uint64_t)sm.inv[w0 + 1] << 32);
+  uint64_t i23 = sm.inv[w0 + 2];
+  if (sizeof(KeyT) == 16) i23 |= (uint64_t)sm.inv[w0 + 3] << 32;
+  const uint64_t vlo = sh0 ? (i01 >> sh0) | (i23 << (64 - sh0)) : i01;
+  const uint64_t vhi = i23 >> sh0;
+  // the bases entering the window at slots 1..ITEMS-1: positions p0 + k on
+  const uint32_t q0 = p0 + (uint32_t)k, wq = rb + (q0 >> 5), sq = 2 * (q0 & 31);
+  const uint64_t c0 = sm.code[wq], c1 = sm.code[wq + 1];
+  const uint64_t nb = sq ? (c0 << sq) | (c1 >> (64 - sq)) : c0;
+  const uint64_t kbits = k >= 64 ? ~0ULL : ((1ULL << k) - 1);  // one bit per base
+  const uint64_t kmask = sizeof(KeyT) == 8 ? (k >= 32 ? ~0ULL : ((1ULL << (2 * k)) - 1))
+                                           : (k >= 64 ? ~0ULL : ((1ULL << (2 * k - 64)) - 1));
+  KeyT fw = window_key<KeyT>(sm.code[w0], sm.code[w0 + 1], sm.code[w0 + 2], 2 * sh0, k);
+  KeyT rc = rd.canonical ? revcomp(fw, k) : fw;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    if (j > 0) {
+      const uint32_t b = (uint32_t)(nb >> (64 - 2 * j)) & 3u;
+      key_push(fw, b, kmask);
+      if (rd.canonical) rc_push(rc, b, k);
+    }
+    const uint64_t win = j ? (vlo >> j) | (vhi << (64 - j)) : vlo;
+    const uint32_t p = p0 + j;
+    bool v = (uint32_t)j < nslots && (win & kbits) == 0 && p + (uint32_t)k <= mp;
+    // kcounter_funcs.cpp:97: the run left open at the end of a record is
+    // emitted only if it starts before READLEN-1, so a lone base in the last
+    // position never forms a 1-mer
+    if (k == 1 && v && p == rd.L - 1 &&
+        (rd.L < 2 || ((sm.inv[rb + ((p - 1) >> 5)] >> ((p - 1) & 31)) & 1)))
+      v = false;
+    KeyT key = rd.canonical ? (key_lt(rc, fw) ? rc : fw) : fw;
+    if (v && rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes) != (uint32_t)rd.owner_bin)
+      v = false;
+    keys[j] = key;
+    vmask |= (v ? 1u : 0u) << j;
+  }
+}
+
+
+// L1: tiles of records -> their k-mers in first-level bucket order, each
+// written contiguously at tile * tcap1 with the tile's bucket starts in off1.
+// Also the record-format checks and the k-mer total.  Persistent: workgroup w
+// runs on XCD w % 8, each XCD owns a contiguous range of tiles (so the offset
+// words of neighbouring tiles meet in one L2) and its workgroups take them in
+// turn, loading the next tile's records while the current one is processed.
 template <typename KeyT, int NT, int ITEMS, bool ONE>
 __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   constexpr int TILE = NT * ITEMS;
+  constexpr int RW = (ExtractSmemBig::RAW_BYTES / 4 + 4 + NT - 1) / NT;
   union Smem {
     ExtractSmemBig ex;
     KeyT stage[TILE];
@@ -682,47 +868,72 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
   __shared__ uint32_t s_job;
-  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t job = ONE ? 0u : find_job_wg(a.pre1, a.njobs, item, &s_job);
-  const JobDesc &J = ONE ? one : a.jobs[job];
-  const ReadsDesc rd = J.rd;
-  const MsdArgs m = J.m;
-  KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
-  uint64_t *n_dev = J.nd;
-  uint32_t *err = J.err;
-  const uint32_t tile = item - (ONE ? 0u : a.pre1[job]);
-  for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
-  KeyT keys[ITEMS];
-  uint32_t vmask, dig[ITEMS], rank[ITEMS];
-  extract_tile<KeyT, NT, ITEMS, true>(rd, tile, sm.ex, keys, vmask, err);
+  const uint32_t nitems = ONE ? one.m.ntiles1 : a.pre1[a.njobs];
+  const uint32_t G = gridDim.x, x = blockIdx.x & 7, nx = (G - x + 7) >> 3;
+  const uint32_t per = nitems >> 3, rem = nitems & 7;
+  const uint32_t lo = x * per + (x < rem ? x : rem);
+  const uint32_t hi = lo + per + (x < rem ? 1u : 0u);
+  uint32_t item = lo + (blockIdx.x >> 3);
+  uint32_t job = 0;
+  uint32_t wv[RW];
+  if (item < hi) {
+    if (!ONE) job = find_job_wg(a.pre1, a.njobs, item, &s_job);
+    const JobDesc &Jf = ONE ? one : a.jobs[job];
+    l1_load_raw<NT, RW>(Jf.rd, item - (ONE ? 0u : a.pre1[job]), wv);
+  }
+  for (; item < hi; item += nx) {
+    const JobDesc &J = ONE ? one : a.jobs[job];
+    const ReadsDesc rd = J.rd;
+    const MsdArgs m = J.m;
+    KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
+    uint64_t *n_dev = J.nd;
+    uint32_t *err = J.err;
+    const uint32_t tile = item - (ONE ? 0u : a.pre1[job]);
+    const uint32_t tid = opaque_tid();
+    __syncthreads();  // the previous tile is written out: the LDS is free
+    for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
+    l1_stage_raw<NT, RW>(rd, tile, wv, sm.ex);
+    // the next tile's records are loaded while this one is processed
+    const uint32_t nitem = item + nx;
+    if (nitem < hi) {
+      if (!ONE) job = find_job_wg(a.pre1, a.njobs, nitem, &s_job);
+      const JobDesc &Jn = ONE ? one : a.jobs[job];
+      l1_load_raw<NT, RW>(Jn.rd, nitem - (ONE ? 0u : a.pre1[job]), wv);
+    }
+    KeyT keys[ITEMS];
+    uint32_t vmask, dig[ITEMS], rank[ITEMS];
+    l1_windows<KeyT, NT, ITEMS>(rd, tile, sm.ex, keys, vmask, err);
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    if ((vmask >> j) & 1) {
-      dig[j] = digit_eval(m.d1, keys[j]);
-      rank[j] = atomicAdd(&s_h[dig[j]], 1u);
+    for (int j = 0; j < ITEMS; ++j) {
+      if ((vmask >> j) & 1) {
+        dig[j] = digit_eval(m.d1, keys[j]);
+        rank[j] = atomicAdd(&s_h[dig[j]], 1u);
+      }
+    }
+    __syncthreads();
+    scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
+    const uint32_t total = s_h[m.nb1];
+    uint16_t *off = m.off1 + tile;
+    for (uint32_t d = tid; d <= m.nb1; d += NT) off[(uint64_t)d * m.ntiles1] = (uint16_t)s_h[d];
+    if (tid == 0 && total)
+      atomicAdd(reinterpret_cast<unsigned long long *>(n_dev), (unsigned long long)total);
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+      if ((vmask >> j) & 1) sm.stage[s_h[dig[j]] + rank[j]] = keys[j];
+    __syncthreads();
+    KeyT *o = out1 + (uint64_t)tile * m.tcap1;
+    constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;  // LDS reads of a chunk, then its stores
+#pragma unroll
+    for (int c0 = 0; c0 < ITEMS; c0 += CH) {
+      KeyT w[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (c0 + i < ITEMS && tid + (c0 + i) * NT < total) w[i] = sm.stage[tid + (c0 + i) * NT];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (c0 + i < ITEMS && tid + (c0 + i) * NT < total) o[tid + (c0 + i) * NT] = w[i];
     }
   }
-  __syncthreads();
-  scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
-  const uint32_t total = s_h[m.nb1];
-  uint16_t *off = m.off1 + tile;
-  for (uint32_t d = tid; d <= m.nb1; d += NT) off[(uint64_t)d * m.ntiles1] = (uint16_t)s_h[d];
-  if (tid == 0 && total) atomicAdd(reinterpret_cast<unsigned long long *>(n_dev), (unsigned long long)total);
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j)
-    if ((vmask >> j) & 1) sm.stage[s_h[dig[j]] + rank[j]] = keys[j];
-  __syncthreads();
-  KeyT *o = out1 + (uint64_t)tile * m.tcap1;
-  KeyT w[ITEMS];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if (tid + u * NT < total) w[u] = sm.stage[tid + u * NT];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if (tid + u * NT < total) o[tid + u * NT] = w[u];
-  (void)lane;
-  (void)wave;
 }
 
 // L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
@@ -1200,10 +1411,24 @@ void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const 
     k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
 }
 
+// compute units of the stream's device (one persistent L1 workgroup each)
+static uint32_t device_cus(hipStream_t st) {
+  static uint32_t cache[64] = {};
+  int dev = 0;
+  if (hipStreamGetDevice(st, &dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cache[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cache[dev] = (uint32_t)n;
+  }
+  return cache[dev];
+}
+
 template <typename KeyT, int ITEMS, int NT3, int NB3, int HT, bool ONE>
 static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint32_t n2, uint32_t n3,
                        hipStream_t st) {
-  if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<n1, MSD_NT, 0, st>>>(a, one);
+  if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<std::min(n1, device_cus(st)), MSD_NT, 0, st>>>(a, one);
   if (n2) k_l2<KeyT, MSD_NT, ITEMS, ONE><<<n2, MSD_NT, 0, st>>>(a, one);
   constexpr int FI = sizeof(KeyT) == 8 ? FIN_ITEMS_1 : FIN_ITEMS_2;
   if (n3) k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<n3, NT3, 0, st>>>(a, one);

@@ -677,9 +677,10 @@ DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, 
 // is the previous one shifted by one base (kcounter_funcs.cpp:25-38 does the
 // same per read).
 
-// Raw words of a tile's records, loaded into registers (every load in flight
-// at once); word i of the tile starts at byte a0 + 4i, a0 = first record
-// start rounded down to 4.
+// s_waitcnt operand that waits for every vector memory operation (vmcnt 0)
+// and for nothing else (expcnt 7, lgkmcnt 15)
+constexpr int VMCNT0 = 0x0F70;
+
 // The thread index, opaque to the compiler: per-thread address arithmetic is
 // recomputed in each tile of a persistent loop instead of being hoisted out
 // of it and spilled.
@@ -689,31 +690,52 @@ DK_DEV uint32_t opaque_tid() {
   return t;
 }
 
-template <int NT, int RW>
-DK_DEV void l1_load_raw(const ReadsDesc &rd, uint32_t tile, uint32_t (&wv)[RW]) {
-  const uint32_t tid = opaque_tid();
-  const uint64_t r0 = (uint64_t)tile * rd.RT;
-  const uint64_t left = rd.nrec - r0;
-  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
-  const uint64_t b0 = r0 * rd.stride;
-  uint64_t b1 = b0 + (uint64_t)nr * rd.stride;
+// Geometry of a tile's raw bytes: records r0 .. r0 + nr - 1 of the job start
+// at byte b0; word i of the tile is the 4 bytes at a0 + 4i, a0 = b0 rounded
+// down to 4, nwords of them.
+struct L1Span {
+  uint64_t r0, b0, a0;
+  uint32_t nr, nwords;
+};
+DK_DEV L1Span l1_span(const ReadsDesc &rd, uint32_t tile) {
+  L1Span t;
+  t.r0 = (uint64_t)tile * rd.RT;
+  const uint64_t left = rd.nrec - t.r0;
+  t.nr = (uint32_t)(left < rd.RT ? left : rd.RT);
+  t.b0 = t.r0 * rd.stride;
+  uint64_t b1 = t.b0 + (uint64_t)t.nr * rd.stride;
   if (b1 > rd.nbytes) b1 = rd.nbytes;
-  const uint64_t a0 = b0 & ~3ULL;
-  const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
-  const uint32_t *src32 = reinterpret_cast<const uint32_t *>(rd.reads + a0);
+  t.a0 = t.b0 & ~3ULL;
+  t.nwords = (uint32_t)((b1 - t.a0 + 3) >> 2);
+  return t;
+}
+
+// Start loading a tile's raw words straight into LDS (raw[i] = word i, no
+// registers involved; the loads complete under vmcnt).  A word that would
+// read past the job's end is left out: l1_patch_tail assembles it.
+template <int NT, int RAWW>
+DK_DEV void l1_prefetch_raw(const ReadsDesc &rd, uint32_t tile, uint32_t *raw) {
+  const uint32_t tid = opaque_tid(), wbase = tid & ~63u;
+  const L1Span t = l1_span(rd, tile);
+  const uint32_t *src32 = reinterpret_cast<const uint32_t *>(rd.reads + t.a0);
 #pragma unroll
-  for (int j = 0; j < RW; ++j) {
+  for (int j = 0; j < (RAWW + NT - 1) / NT; ++j) {
     const uint32_t i = tid + j * NT;
-    wv[j] = 0;
-    if (i < nwords && a0 + 4ULL * i + 4 <= rd.nbytes) wv[j] = src32[i];
+    if (i < t.nwords && t.a0 + 4ULL * i + 4 <= rd.nbytes)
+      __builtin_amdgcn_global_load_lds(src32 + i, raw + j * NT + wbase, 4, 0, 0);
   }
-#pragma unroll
-  for (int j = 0; j < RW; ++j) {
-    const uint32_t i = tid + j * NT;
-    const uint64_t addr = a0 + 4ULL * i;
-    if (i < nwords && addr + 4 > rd.nbytes)  // the job's last bytes: not a whole word
-      for (uint32_t b = 0; b < 4; ++b)
-        if (addr + b < rd.nbytes) wv[j] |= (uint32_t)rd.reads[addr + b] << (8 * b);
+}
+
+// The job's last word, when the job does not end on a 4-byte boundary.
+DK_DEV void l1_patch_tail(const ReadsDesc &rd, uint32_t tile, uint32_t *raw) {
+  const L1Span t = l1_span(rd, tile);
+  const uint32_t i = t.nwords - 1;
+  const uint64_t addr = t.a0 + 4ULL * i;
+  if (threadIdx.x == 0 && t.nwords && addr + 4 > rd.nbytes) {
+    uint32_t w = 0;
+    for (uint32_t b = 0; b < 4; ++b)
+      if (addr + b < rd.nbytes) w |= (uint32_t)rd.reads[addr + b] << (8 * b);
+    raw[i] = w;
   }
 }
 
@@ -730,45 +752,36 @@ DK_DEV void rc_push(K2 &x, uint32_t b, int k) {  // 33 <= k <= 64: the new base 
   x.hi = (x.hi >> 2) | ((uint64_t)(b ^ 3u) << (2 * k - 66));
 }
 
-// l1_stage_raw puts the tile held in wv into LDS; l1_windows decodes it
-// (records r0 .. r0 + nr - 1 of the job) into 2-bit codes, invalid-base
-// masks and first-'M' positions, then builds this thread's ITEMS windows.  Restates kcounter_funcs.cpp:40-111: a window
-// is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it starts
-// before the record's first 'M' (0xF0 ends the read).  Ends with the
-// thread's keys; the LDS is still read by other threads.
-template <int NT, int RW, typename SM>
-DK_DEV void l1_stage_raw(const ReadsDesc &rd, uint32_t tile, const uint32_t (&wv)[RW], SM &sm) {
-  const uint32_t tid = opaque_tid();
-  const uint64_t r0 = (uint64_t)tile * rd.RT;
-  const uint64_t left = rd.nrec - r0;
-  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
-  const uint64_t b0 = r0 * rd.stride;
-  uint64_t b1 = b0 + (uint64_t)nr * rd.stride;
-  if (b1 > rd.nbytes) b1 = rd.nbytes;
-  const uint64_t a0 = b0 & ~3ULL;
-  const uint32_t nwords = (uint32_t)((b1 - a0 + 3) >> 2);
-#pragma unroll
-  for (int j = 0; j < RW; ++j) {
-    const uint32_t i = tid + j * NT;
-    if (i < nwords) sm.raw[i] = wv[j];
-  }
-  for (uint32_t r = tid; r < nr; r += NT) sm.mpos[r] = rd.L;
-}
+// LDS of an L1 tile's decoded records: packed 2-bit codes, masks of positions
+// that cannot be in a k-mer, and each record's first 'M'.
+struct L1Code {
+  uint64_t code[L1_CODEW];  // MSB-first 2-bit codes, 32 bases per word
+  uint32_t inv[L1_CODEW];   // LSB-first: bit q set = base 32w+q cannot be in a k-mer
+  uint32_t mpos[L1_MAXRT];  // first 'M' (end of read) per record, L if none
+};
 
-template <typename KeyT, int NT, int ITEMS, typename SM>
-DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[ITEMS],
-                       uint32_t &vmask, uint32_t *err) {
+// Decode the tile's raw records (in `raw`, records r0 .. r0 + nr - 1 of the
+// job) into 2-bit codes, invalid-base masks and first-'M' positions in LDS,
+// then build this thread's ITEMS windows.  Restates kcounter_funcs.cpp:40-111:
+// a window is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it
+// starts before the record's first 'M' (0xF0 ends the read).  The caller has
+// set mpos to L and made raw and mpos visible (barrier); after the decode's
+// closing barrier `after_decode` runs, and raw is no longer read.
+template <typename KeyT, int NT, int ITEMS, typename AfterDecode>
+DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32, L1Code &sm,
+                       KeyT (&keys)[ITEMS], uint32_t &vmask, uint32_t *err,
+                       AfterDecode after_decode) {
   const uint32_t tid = opaque_tid();
-  const uint64_t r0 = (uint64_t)tile * rd.RT;
-  const uint64_t left = rd.nrec - r0;
-  const uint32_t nr = (uint32_t)(left < rd.RT ? left : rd.RT);
-  const uint64_t b0 = r0 * rd.stride;
-  const uint64_t a0 = b0 & ~3ULL;
-  __syncthreads();  // the raw records are in LDS
-
-  const uint8_t *raw = reinterpret_cast<const uint8_t *>(sm.raw) + (b0 - a0);
+  const L1Span t = l1_span(rd, tile);
+  const uint32_t nr = t.nr;
+  const uint64_t b0 = t.b0;
+  const uint8_t *raw = reinterpret_cast<const uint8_t *>(raw32) + (b0 - t.a0);
   const uint32_t wpl = (rd.L + 31) >> 5;
-  for (uint32_t it = tid; it < nr * rd.wpr; it += NT) {
+  static_assert(L1_CODEW <= 2 * NT, "two code words per thread at most");
+#pragma nounroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const uint32_t it = tid + pass * NT;
+    if (it >= nr * rd.wpr) break;
     const uint32_t r = it / rd.wpr, w = it - r * rd.wpr;
     uint64_t code = 0;
     uint32_t inv = 0xFFFFFFFFu;
@@ -783,8 +796,8 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[
         const uint32_t b = j < rd.L ? rec[j] : 0u;
         const uint32_t u = b & 0xDFu;  // upper-case letters
         const bool ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T');
-        const uint32_t t = (b >> 1) & 3u;        // A0 C1 T2 G3 (ASCII bits 1-2)
-        const uint32_t c = t ^ ((t >> 1) ^ 1u);  // -> C0 A1 T2 G3
+        const uint32_t tt = (b >> 1) & 3u;          // A0 C1 T2 G3 (ASCII bits 1-2)
+        const uint32_t c = tt ^ ((tt >> 1) ^ 1u);  // -> C0 A1 T2 G3
         code = (code << 2) | (ok ? c : 0u);
         inv |= (ok ? 0u : 1u) << q;
         if (u == 'M' && mloc == 0xFFFFFFFFu) mloc = j;
@@ -798,6 +811,7 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[
     sm.inv[it] = inv;
   }
   __syncthreads();
+  after_decode();  // raw is free from here on
 
   vmask = 0;
 #pragma unroll
@@ -849,25 +863,27 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys)[
   }
 }
 
-
 // L1: tiles of records -> their k-mers in first-level bucket order, each
 // written contiguously at tile * tcap1 with the tile's bucket starts in off1.
 // Also the record-format checks and the k-mer total.  Persistent: workgroup w
 // runs on XCD w % 8, each XCD owns a contiguous range of tiles (so the offset
 // words of neighbouring tiles meet in one L2) and its workgroups take them in
-// turn, loading the next tile's records while the current one is processed.
+// turn; the next tile's records stream into LDS while the current one is
+// processed.
 template <typename KeyT, int NT, int ITEMS, bool ONE>
 __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   constexpr int TILE = NT * ITEMS;
-  constexpr int RW = (ExtractSmemBig::RAW_BYTES / 4 + 4 + NT - 1) / NT;
+  constexpr int RAWW = L1_RAW / 4 + 4;  // raw words of a tile (records + alignment)
   union Smem {
-    ExtractSmemBig ex;
+    L1Code ex;
     KeyT stage[TILE];
   };
   __shared__ Smem sm;
+  __shared__ uint32_t s_raw[RAWW];
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total;
   __shared__ uint32_t s_job;
+  static_assert(NT >= 1024 && L1_MAXRT <= NT, "one histogram slot and record per thread");
   const uint32_t nitems = ONE ? one.m.ntiles1 : a.pre1[a.njobs];
   const uint32_t G = gridDim.x, x = blockIdx.x & 7, nx = (G - x + 7) >> 3;
   const uint32_t per = nitems >> 3, rem = nitems & 7;
@@ -875,12 +891,12 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   const uint32_t hi = lo + per + (x < rem ? 1u : 0u);
   uint32_t item = lo + (blockIdx.x >> 3);
   uint32_t job = 0;
-  uint32_t wv[RW];
   if (item < hi) {
     if (!ONE) job = find_job_wg(a.pre1, a.njobs, item, &s_job);
     const JobDesc &Jf = ONE ? one : a.jobs[job];
-    l1_load_raw<NT, RW>(Jf.rd, item - (ONE ? 0u : a.pre1[job]), wv);
+    l1_prefetch_raw<NT, RAWW>(Jf.rd, item - (ONE ? 0u : a.pre1[job]), s_raw);
   }
+  __builtin_amdgcn_s_waitcnt(VMCNT0);
   for (; item < hi; item += nx) {
     const JobDesc &J = ONE ? one : a.jobs[job];
     const ReadsDesc rd = J.rd;
@@ -890,36 +906,44 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
     uint32_t *err = J.err;
     const uint32_t tile = item - (ONE ? 0u : a.pre1[job]);
     const uint32_t tid = opaque_tid();
-    __syncthreads();  // the previous tile is written out: the LDS is free
-    for (uint32_t i = tid; i <= m.nb1; i += NT) s_h[i] = 0;
-    l1_stage_raw<NT, RW>(rd, tile, wv, sm.ex);
-    // the next tile's records are loaded while this one is processed
-    const uint32_t nitem = item + nx;
-    if (nitem < hi) {
-      if (!ONE) job = find_job_wg(a.pre1, a.njobs, nitem, &s_job);
-      const JobDesc &Jn = ONE ? one : a.jobs[job];
-      l1_load_raw<NT, RW>(Jn.rd, nitem - (ONE ? 0u : a.pre1[job]), wv);
-    }
+    __syncthreads();  // the previous tile is written out; this tile's records are in LDS
+    s_h[tid] = 0;
+    if (tid == 0) s_h[1024] = 0;
+    if (tid < rd.RT) sm.ex.mpos[tid] = rd.L;
+    l1_patch_tail(rd, tile, s_raw);
+    __syncthreads();
     KeyT keys[ITEMS];
-    uint32_t vmask, dig[ITEMS], rank[ITEMS];
-    l1_windows<KeyT, NT, ITEMS>(rd, tile, sm.ex, keys, vmask, err);
+    uint32_t vmask, dr[ITEMS];  // digit << 16 | rank within the digit (< 2^14 keys per tile)
+    l1_windows<KeyT, NT, ITEMS>(rd, tile, s_raw, sm.ex, keys, vmask, err, [&]() {
+      // the records are decoded: the next tile's stream into the raw buffer
+      const uint32_t nitem = item + nx;
+      if (nitem < hi) {
+        if (!ONE) job = find_job_wg(a.pre1, a.njobs, nitem, &s_job);
+        const JobDesc &Jn = ONE ? one : a.jobs[job];
+        l1_prefetch_raw<NT, RAWW>(Jn.rd, nitem - (ONE ? 0u : a.pre1[job]), s_raw);
+      }
+    });
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if ((vmask >> j) & 1) {
-        dig[j] = digit_eval(m.d1, keys[j]);
-        rank[j] = atomicAdd(&s_h[dig[j]], 1u);
+        const uint32_t d = digit_eval(m.d1, keys[j]);
+        dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
       }
     }
     __syncthreads();
     scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
     const uint32_t total = s_h[m.nb1];
+    // the next tile's records arrived while this one was processed: settle
+    // them before this tile's stores, so the next iteration waits for nothing
+    __builtin_amdgcn_s_waitcnt(VMCNT0);
     uint16_t *off = m.off1 + tile;
-    for (uint32_t d = tid; d <= m.nb1; d += NT) off[(uint64_t)d * m.ntiles1] = (uint16_t)s_h[d];
+    if (tid <= m.nb1) off[(uint64_t)tid * m.ntiles1] = (uint16_t)s_h[tid];
+    if (tid == 0 && m.nb1 >= NT) off[(uint64_t)NT * m.ntiles1] = (uint16_t)s_h[NT];
     if (tid == 0 && total)
       atomicAdd(reinterpret_cast<unsigned long long *>(n_dev), (unsigned long long)total);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-      if ((vmask >> j) & 1) sm.stage[s_h[dig[j]] + rank[j]] = keys[j];
+      if ((vmask >> j) & 1) sm.stage[s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = keys[j];
     __syncthreads();
     KeyT *o = out1 + (uint64_t)tile * m.tcap1;
     constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;  // LDS reads of a chunk, then its stores

@@ -331,7 +331,8 @@ int plan_choose_path(Plan *pl, int algo) {
   if (algo == DAKC_ALGO_LSD || p.nmax == 0) return 0;
   const uint32_t bits = 2 * (uint32_t)p.k;
   const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : p.tune_fin_nt == 1024 ? 1024u : 512u;
-  const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap2 = (uint64_t)msd_tile(p.kw);
+  const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap1 = (uint64_t)msd_tile(p.kw),
+                 cap2 = (uint64_t)l2_cap(p.kw);
   const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 60) / 100;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
@@ -346,12 +347,12 @@ int plan_choose_path(Plan *pl, int algo) {
   // L1 tile geometry: a record's slots take ceil(S / items) threads
   const uint32_t wpr = (p.L + 31) / 32 + 2;
   const uint32_t l1_items = p.kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2;
-  uint32_t rt = p.S ? (uint32_t)MSD_NT / ((p.S + l1_items - 1) / l1_items) : (uint32_t)cap2;
+  uint32_t rt = p.S ? (uint32_t)MSD_NT / ((p.S + l1_items - 1) / l1_items) : (uint32_t)cap1;
   rt = std::min<uint32_t>(rt, L1_MAXRT);
   rt = std::min<uint32_t>(rt, L1_RAW / p.stride);
   rt = std::min<uint32_t>(rt, L1_CODEW / wpr);
   rt = std::max<uint32_t>(rt, 1);
-  if ((uint64_t)rt * p.S > cap2) return 0;
+  if ((uint64_t)rt * p.S > cap1) return 0;
   p.b1 = b1;
   p.b2 = b2;
   p.b3 = std::min<uint32_t>(fin_nt >= 512 ? 10 : 9, bits - b1 - b2);
@@ -365,7 +366,7 @@ int plan_choose_path(Plan *pl, int algo) {
   p.tcap1 = rt * p.S;
   const double per_tile_bucket = (double)p.nmax / p.ntiles1 / p.nb1;
   uint32_t G = (uint32_t)std::max(1.0, std::floor(0.75 * (double)cap2 / std::max(per_tile_bucket, 1e-9)));
-  G = std::min<uint32_t>(G, MSD_NT);
+  G = std::min<uint32_t>(G, L2_NT);
   G = std::min<uint32_t>(G, p.ntiles1);
   p.G = std::max<uint32_t>(G, 1);
   p.ngroups = (p.ntiles1 + p.G - 1) / p.G;

@@ -76,6 +76,10 @@ constexpr int MSD_NT = 1024;
 constexpr int MSD_ITEMS_1 = 15;
 constexpr int MSD_ITEMS_2 = 8;
 inline int msd_tile(int kw) { return MSD_NT * (kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2); }
+// L2 workgroups (NT threads; 1024 measured faster than two 512s per CU) and
+// the keys one holds
+constexpr int L2_NT = 1024;
+inline int l2_cap(int kw) { return L2_NT * (kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2); }
 constexpr int L1_RAW = 20480;    // LDS bytes of raw records per L1 tile
 constexpr int L1_CODEW = 1792;   // packed code words per L1 tile
 constexpr int L1_MAXRT = 256;    // records per L1 tile

@@ -605,14 +605,13 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
   pre += x - (((uint64_t)cnt << 32) | sum);
   __syncthreads();
   const uint32_t tot = *s_total;
-  mine = 0;
-  if (tot > CAP) return tot;
+  const bool fits = tot <= CAP;  // uniform
   uint32_t pos = (uint32_t)pre, k = (uint32_t)(pre >> 32);
   if (tid == 0) g.wpre[0] = 0;
 #pragma unroll
   for (int u = 0; u < TPT; ++u) {
     const uint32_t q = tid * TPT + u;
-    if (lenv[u]) {
+    if (fits && lenv[u]) {
       g.delta[k] = q * stride + o0v[u] - pos;
       atomicOr(&g.bits[pos >> 5], 1u << (pos & 31));
       // words whose first position lies in (pos, pos + len]: k + 1 starts precede them
@@ -622,32 +621,34 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
     }
   }
   __syncthreads();
-  mine = tot > tid ? (tot - tid + NT - 1) / NT : 0;
+  mine = fits && tot > tid ? (tot - tid + NT - 1) / NT : 0;
   if (mine > (uint32_t)ITEMS) mine = ITEMS;
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) v[u] = KeyT{};
   // in chunks of CH keys: the word reads, then the segment-offset reads, then
   // the key loads, so each group of LDS reads waits once and the loads of a
   // chunk are in flight while the next chunk's offsets are found
   constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;
 #pragma unroll
   for (int c0 = 0; c0 < ITEMS; c0 += CH) {
+    // LDS reads unconditional (positions past the total read in-range words
+    // whose values are not used), so each group of reads waits once
     uint32_t wp[CH], wb[CH];
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      const int u = c0 + i;
-      const uint32_t w = (u * NT + tid) >> 5;
-      wp[i] = wb[i] = 0;
-      if (u < ITEMS && (uint32_t)u < mine) {
-        wp[i] = g.wpre[w];
-        wb[i] = g.bits[w];
-      }
+      const uint32_t w = ((c0 + i < ITEMS ? c0 + i : 0) * NT + tid) >> 5;
+      wp[i] = g.wpre[w];
+      wb[i] = g.bits[w];
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep each group of reads together
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      const int u = c0 + i;
-      const uint32_t p = u * NT + tid;
-      if (u < ITEMS && (uint32_t)u < mine)
-        wp[i] = g.delta[wp[i] + __popc(wb[i] & (0xFFFFFFFFu >> (31 - (p & 31)))) - 1] + p;
+      const uint32_t p = (c0 + i < ITEMS ? c0 + i : 0) * NT + tid;
+      uint32_t seg = wp[i] + __popc(wb[i] & (0xFFFFFFFFu >> (31 - (p & 31)))) - 1;
+      seg = seg < (uint32_t)NSEG ? seg : (uint32_t)NSEG - 1;
+      wp[i] = g.delta[seg] + p;
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int u = c0 + i;
@@ -923,11 +924,21 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
         l1_prefetch_raw<NT, RAWW>(Jn.rd, nitem - (ONE ? 0u : a.pre1[job]), s_raw);
       }
     });
+    if (m.d1.num_pes <= 1) {  // first-level digit: the top bits alone
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      if ((vmask >> j) & 1) {
-        const uint32_t d = digit_eval(m.d1, keys[j]);
-        dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+      for (int j = 0; j < ITEMS; ++j) {
+        if ((vmask >> j) & 1) {
+          const uint32_t d = key_bits(keys[j], m.d1.shift, m.d1.nbits);
+          dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+        }
+      }
+    } else {  // owner * 2^b1 + top bits
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        if ((vmask >> j) & 1) {
+          const uint32_t d = digit_eval(m.d1, keys[j]);
+          dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+        }
       }
     }
     __syncthreads();
@@ -1001,7 +1012,7 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     if ((uint32_t)u < mine) {
-      dig[u] = digit_eval(m.d2, v[u]);
+      dig[u] = key_bits(v[u], m.d2.shift, m.d2.nbits);  // levels 2 and 3 split by key bits
       rank[u] = atomicAdd(&s_h[dig[u]], 1u);
     }
   }
@@ -1147,7 +1158,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     ek[j] = tk[tid + j * NT];
     ec[j] = tc[tid + j * NT];
     if (ek[j] != HASH_EMPTY) {
-      ed[j] = digit_eval(m.d3, ek[j]);
+      ed[j] = key_bits(ek[j], m.d3.shift, m.d3.nbits);
       er[j] = atomicAdd(&s_h[ed[j]], 1u);
     }
   }
@@ -1267,7 +1278,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     if ((uint32_t)u < mine) {
-      dig[u] = digit_eval(m.d3, keys[u]);
+      dig[u] = key_bits(keys[u], m.d3.shift, m.d3.nbits);
       rank[u] = atomicAdd(&s_h[dig[u]], 1u);
     }
   }
@@ -1290,7 +1301,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
     if (i < mcnt) {
       const KeyT x = B[i];
       keys[q] = x;
-      dig[q] = digit_eval(m.d3, x);
+      dig[q] = key_bits(x, m.d3.shift, m.d3.nbits);
       if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
     }
   }
@@ -1453,7 +1464,7 @@ template <typename KeyT, int ITEMS, int NT3, int NB3, int HT, bool ONE>
 static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint32_t n2, uint32_t n3,
                        hipStream_t st) {
   if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<std::min(n1, device_cus(st)), MSD_NT, 0, st>>>(a, one);
-  if (n2) k_l2<KeyT, MSD_NT, ITEMS, ONE><<<n2, MSD_NT, 0, st>>>(a, one);
+  if (n2) k_l2<KeyT, L2_NT, ITEMS, ONE><<<n2, L2_NT, 0, st>>>(a, one);
   constexpr int FI = sizeof(KeyT) == 8 ? FIN_ITEMS_1 : FIN_ITEMS_2;
   if (n3) k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<n3, NT3, 0, st>>>(a, one);
 }

@@ -19,6 +19,20 @@ namespace dk {
 
 // ------------------------------------------------------------- utilities --
 
+// s_waitcnt operand that waits for every vector memory operation (vmcnt 0)
+// and for nothing else (expcnt 7, lgkmcnt 15)
+constexpr int VMCNT0 = 0x0F70;
+
+// The thread index, opaque to the compiler: per-thread address arithmetic is
+// recomputed in each tile of a persistent loop instead of being hoisted out
+// of it and spilled.
+DK_DEV uint32_t opaque_tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -521,7 +535,7 @@ __global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uin
 // exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = total
 template <int NT>
 DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_total) {
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = opaque_tid();
   uint32_t v[4], sum = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -574,7 +588,7 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
   constexpr uint32_t CAP = NT * ITEMS, NW = GatherSmem<NSEG, NT * ITEMS>::NW;
   static_assert(NT * TPT >= NSEG, "segments per thread");
   static_assert(NW >= NT / 32, "scan scratch");
-  const uint32_t tid = threadIdx.x, lane = __lane_id();
+  const uint32_t tid = opaque_tid(), lane = tid & 63;
   uint32_t o0v[TPT], lenv[TPT], sum = 0, cnt = 0;
 #pragma unroll
   for (int u = 0; u < TPT; ++u) {
@@ -677,19 +691,6 @@ DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, 
 // so a thread's windows slide along one record: after the first, each k-mer
 // is the previous one shifted by one base (kcounter_funcs.cpp:25-38 does the
 // same per read).
-
-// s_waitcnt operand that waits for every vector memory operation (vmcnt 0)
-// and for nothing else (expcnt 7, lgkmcnt 15)
-constexpr int VMCNT0 = 0x0F70;
-
-// The thread index, opaque to the compiler: per-thread address arithmetic is
-// recomputed in each tile of a persistent loop instead of being hoisted out
-// of it and spilled.
-DK_DEV uint32_t opaque_tid() {
-  uint32_t t = threadIdx.x;
-  asm volatile("" : "+v"(t));
-  return t;
-}
 
 // Geometry of a tile's raw bytes: records r0 .. r0 + nr - 1 of the job start
 // at byte b0; word i of the tile is the 4 bytes at a0 + 4i, a0 = b0 rounded
@@ -973,64 +974,106 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
 
 // L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
 // digit order, written at (b * ngroups + g) * cap2, with the digit starts in
-// off2[b][digit][g].
+// off2[b][digit][g].  Items are numbered g * nb1 + b.  Persistent like k_l1:
+// each XCD owns a contiguous range of items, so the workgroups of an XCD take
+// consecutive buckets of one group, whose segments are neighbours in every L1
+// tile (the cache lines they share stay in that XCD's L2).  The next item's
+// keys are gathered into a second register set while the current item is
+// split and written.
 template <typename KeyT, int NT, int ITEMS, bool ONE>
 __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
   constexpr int CAP = NT * ITEMS;
+  static_assert(NT >= 1024, "one second-level digit (nd2 <= 1024) per thread");
   __shared__ KeyT stage[CAP];
   __shared__ GatherSmem<NT, CAP> s_g;
   __shared__ uint32_t s_h[1024 + 1];
-  __shared__ uint32_t s_wave[NT / 64], s_total;
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_gwave[NT / 64], s_gtotal;
   __shared__ uint32_t s_job;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t job = ONE ? 0u : find_job_wg(a.pre2, a.njobs, item, &s_job);
-  const JobDesc &J = ONE ? one : a.jobs[job];
-  const MsdArgs m = J.m;
-  const KeyT *in1 = reinterpret_cast<const KeyT *>(J.out1);
-  KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
-  uint32_t *err = J.err;
-  const uint32_t wg = item - (ONE ? 0u : a.pre2[job]);
-  // consecutive workgroups (one XCD's range) take consecutive buckets of the
-  // same group: their segments are neighbours in every L1 tile, so the cache
-  // lines they share are still in that XCD's L2
-  const uint32_t g = wg / m.nb1, b = wg - g * m.nb1;
-  const uint32_t t0 = g * m.G;
-  const uint32_t nt = m.ntiles1 - t0 < m.G ? m.ntiles1 - t0 : m.G;
-  const uint16_t *row0 = m.off1 + (uint64_t)b * m.ntiles1 + t0;
-  for (uint32_t i = tid; i <= m.nd2; i += NT) s_h[i] = 0;
-  KeyT v[ITEMS];
-  uint32_t mine;
-  uint32_t tot = gather_segments<KeyT, NT, ITEMS, 1, NT>(in1, row0, row0 + m.ntiles1, nt,
-                                                         (uint64_t)t0 * m.tcap1, m.tcap1, s_g,
-                                                         s_wave, &s_total, v, mine);
-  if (tot > (uint32_t)CAP) {
-    if (tid == 0) atomicOr(err, ERR_OVERFLOW);
-    tot = 0;
-  }
-  uint32_t dig[ITEMS], rank[ITEMS];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    if ((uint32_t)u < mine) {
-      dig[u] = key_bits(v[u], m.d2.shift, m.d2.nbits);  // levels 2 and 3 split by key bits
-      rank[u] = atomicAdd(&s_h[dig[u]], 1u);
+  const uint32_t nitems = ONE ? one.m.nb1 * one.m.ngroups : a.pre2[a.njobs];
+  const uint32_t G = gridDim.x, x = blockIdx.x & 7, nx = (G - x + 7) >> 3;
+  const uint32_t per = nitems >> 3, rem = nitems & 7;
+  const uint32_t lo = x * per + (x < rem ? x : rem);
+  const uint32_t hi = lo + per + (x < rem ? 1u : 0u);
+  uint32_t item = lo + (blockIdx.x >> 3);
+
+  // gather item `it` into dst (loads left in flight); returns the key count
+  auto gather_item = [&](uint32_t it, uint32_t &jb, KeyT (&dst)[ITEMS], uint32_t &mn) -> uint32_t {
+    if (!ONE) jb = find_job_wg(a.pre2, a.njobs, it, &s_job);
+    const JobDesc &Jg = ONE ? one : a.jobs[jb];
+    const MsdArgs mg = Jg.m;
+    const uint32_t wgi = it - (ONE ? 0u : a.pre2[jb]);
+    const uint32_t gg = wgi / mg.nb1, bb = wgi - gg * mg.nb1;
+    const uint32_t t0 = gg * mg.G;
+    const uint32_t nt = mg.ntiles1 - t0 < mg.G ? mg.ntiles1 - t0 : mg.G;
+    const uint16_t *row0 = mg.off1 + (uint64_t)bb * mg.ntiles1 + t0;
+    uint32_t t = gather_segments<KeyT, NT, ITEMS, 1, NT>(
+        reinterpret_cast<const KeyT *>(Jg.out1), row0, row0 + mg.ntiles1, nt,
+        (uint64_t)t0 * mg.tcap1, mg.tcap1, s_g, s_gwave, &s_gtotal, dst, mn);
+    if (t > (uint32_t)CAP) {
+      if (threadIdx.x == 0) atomicOr(Jg.err, ERR_OVERFLOW);
+      t = 0;
     }
+    return t;
+  };
+
+  KeyT v[ITEMS];
+  uint32_t job = 0, mine = 0, tot = 0;
+  if (item < hi) tot = gather_item(item, job, v, mine);
+  __builtin_amdgcn_s_waitcnt(VMCNT0);
+  for (; item < hi; item += nx) {
+    const JobDesc &J = ONE ? one : a.jobs[job];
+    const MsdArgs m = J.m;
+    KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
+    const uint32_t wg = item - (ONE ? 0u : a.pre2[job]);
+    const uint32_t g = wg / m.nb1, b = wg - g * m.nb1;
+    const uint32_t tid = opaque_tid();
+    s_h[tid] = 0;  // the previous item's split is done (its stage write-out follows a barrier)
+    if (tid == 0) s_h[1024] = 0;
+    __syncthreads();
+    uint32_t dr[ITEMS];  // digit << 16 | rank within the digit (< 2^14 keys per item)
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      if ((uint32_t)u < mine) {
+        const uint32_t d = key_bits(v[u], m.d2.shift, m.d2.nbits);  // levels 2 and 3 split by key bits
+        dr[u] = (d << 16) | atomicAdd(&s_h[d], 1u);
+      }
+    }
+    // the next item's keys stream in while this one is split; the gather's
+    // barriers also order the ranking before the scan
+    KeyT vn[ITEMS];
+    uint32_t jn = job, mn = 0, tn = 0;
+    const uint32_t nitem = item + nx;
+    if (nitem < hi) tn = gather_item(nitem, jn, vn, mn);
+    else __syncthreads();
+    scan_lds<NT>(s_h, m.nd2, s_wave, &s_total);
+    uint16_t *off = m.off2 + (uint64_t)b * (m.nd2 + 1) * m.ngroups + g;
+    if (tid <= m.nd2) off[(uint64_t)tid * m.ngroups] = (uint16_t)(tot ? s_h[tid] : 0);
+    if (tid == 0 && m.nd2 >= NT) off[(uint64_t)NT * m.ngroups] = (uint16_t)(tot ? s_h[NT] : 0);
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u)
+      if ((uint32_t)u < mine) stage[s_h[dr[u] >> 16] + (dr[u] & 0xFFFFu)] = v[u];
+    __syncthreads();
+    // the next item's keys have arrived: settle them before this item's
+    // stores, so the next iteration waits for nothing
+    __builtin_amdgcn_s_waitcnt(VMCNT0);
+    KeyT *o = out2 + ((uint64_t)b * m.ngroups + g) * m.cap2;
+    constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;
+#pragma unroll
+    for (int c0 = 0; c0 < ITEMS; c0 += CH) {
+      KeyT w[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (c0 + i < ITEMS && tid + (c0 + i) * NT < tot) w[i] = stage[tid + (c0 + i) * NT];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (c0 + i < ITEMS && tid + (c0 + i) * NT < tot) o[tid + (c0 + i) * NT] = w[i];
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) v[u] = vn[u];
+    job = jn;
+    mine = mn;
+    tot = tn;
   }
-  __syncthreads();
-  scan_lds<NT>(s_h, m.nd2, s_wave, &s_total);
-  uint16_t *off = m.off2 + (uint64_t)b * (m.nd2 + 1) * m.ngroups + g;
-  for (uint32_t d = tid; d <= m.nd2; d += NT) off[(uint64_t)d * m.ngroups] = (uint16_t)(tot ? s_h[d] : 0);
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if ((uint32_t)u < mine) stage[s_h[dig[u]] + rank[u]] = v[u];
-  __syncthreads();
-  KeyT *o = out2 + ((uint64_t)b * m.ngroups + g) * m.cap2;
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if (tid + u * NT < tot) v[u] = stage[tid + u * NT];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if (tid + u * NT < tot) o[tid + u * NT] = v[u];
 }
 
 // Exclusive prefix of sub-bucket `idx` over all earlier ones, one wavefront
@@ -1464,7 +1507,7 @@ template <typename KeyT, int ITEMS, int NT3, int NB3, int HT, bool ONE>
 static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint32_t n2, uint32_t n3,
                        hipStream_t st) {
   if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<std::min(n1, device_cus(st)), MSD_NT, 0, st>>>(a, one);
-  if (n2) k_l2<KeyT, L2_NT, ITEMS, ONE><<<n2, L2_NT, 0, st>>>(a, one);
+  if (n2) k_l2<KeyT, L2_NT, ITEMS, ONE><<<std::min(n2, device_cus(st)), L2_NT, 0, st>>>(a, one);
   constexpr int FI = sizeof(KeyT) == 8 ? FIN_ITEMS_1 : FIN_ITEMS_2;
   if (n3) k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<n3, NT3, 0, st>>>(a, one);
 }

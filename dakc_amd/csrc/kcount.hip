@@ -49,7 +49,7 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
 template <int NT>
 DK_DEV uint32_t scan256(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
   static_assert(NT >= 256, "needs 256 threads");
-  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   uint32_t x = tid < 256 ? v : 0;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -69,7 +69,7 @@ DK_DEV uint32_t scan256(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
 // Exclusive scan over all NT threads of one u32 each.
 template <int NT>
 DK_DEV uint32_t scan_block(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
-  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   uint32_t x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -572,7 +572,8 @@ struct GatherSmem {
 };
 
 // Gather nseg segments into the workgroup's registers.  Segment q holds
-// row1[q] - row0[q] keys at src[region0 + q * stride + row0[q]]; q starts at
+// row1[q] - row0[q] keys at src[region0 + q * stride + row0[q]] (the bounds
+// come in registers, from load_rows); q starts at
 // position pref[q] of the concatenation.  Thread t receives positions
 // u * NT + t (u < ITEMS), so each load instruction of a wavefront reads 64
 // consecutive positions, mostly from one segment.  A position's segment is
@@ -580,8 +581,24 @@ struct GatherSmem {
 // and per-word prefix counts (no search).  Returns the total; when it exceeds
 // CAP no key is loaded and mine = 0.  Offsets must fit 32 bits
 // (q * stride + row0[q] < 2^32).
+// Segment bounds of thread t: row0[q], row1[q] for q = t * TPT + u (0 past nseg).
+template <int TPT>
+DK_DEV void load_rows(const uint16_t *row0, const uint16_t *row1, uint32_t nseg, uint32_t (&r0)[TPT],
+                      uint32_t (&r1)[TPT]) {
+  const uint32_t tid = opaque_tid();
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const uint32_t q = tid * TPT + u;
+    r0[u] = r1[u] = 0;
+    if (q < nseg) {
+      r0[u] = row0[q];
+      r1[u] = row1[q];
+    }
+  }
+}
+
 template <typename KeyT, int NT, int ITEMS, int TPT, int NSEG>
-DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uint16_t *row1,
+DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], const uint32_t (&r1)[TPT],
                                 uint32_t nseg, uint64_t region0, uint32_t stride,
                                 GatherSmem<NSEG, NT * ITEMS> &g, uint32_t *s_wave,
                                 uint32_t *s_total, KeyT (&v)[ITEMS], uint32_t &mine) {
@@ -595,8 +612,8 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
     const uint32_t q = tid * TPT + u;
     o0v[u] = lenv[u] = 0;
     if (q < nseg) {
-      o0v[u] = row0[q];
-      lenv[u] = row1[q] - o0v[u];
+      o0v[u] = r0[u];
+      lenv[u] = r1[u] - o0v[u];
       if (lenv[u] > CAP + 1) lenv[u] = CAP + 1;  // keeps the sum in 32 bits
     }
     sum += lenv[u];
@@ -678,7 +695,7 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint16_t *row0, const uin
 // dependent ones.  `item` is uniform; ends with a barrier unless njobs == 1.
 DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, uint32_t *s_job) {
   if (njobs == 1) return 0;
-  for (uint32_t j = threadIdx.x; j < njobs; j += blockDim.x)
+  for (uint32_t j = opaque_tid(); j < njobs; j += blockDim.x)
     if (pre[j] <= item && item < pre[j + 1]) *s_job = j;
   __syncthreads();
   return __builtin_amdgcn_readfirstlane(*s_job);
@@ -733,7 +750,7 @@ DK_DEV void l1_patch_tail(const ReadsDesc &rd, uint32_t tile, uint32_t *raw) {
   const L1Span t = l1_span(rd, tile);
   const uint32_t i = t.nwords - 1;
   const uint64_t addr = t.a0 + 4ULL * i;
-  if (threadIdx.x == 0 && t.nwords && addr + 4 > rd.nbytes) {
+  if (opaque_tid() == 0 && t.nwords && addr + 4 > rd.nbytes) {
     uint32_t w = 0;
     for (uint32_t b = 0; b < 4; ++b)
       if (addr + b < rd.nbytes) w |= (uint32_t)rd.reads[addr + b] << (8 * b);
@@ -1006,8 +1023,10 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     const uint32_t t0 = gg * mg.G;
     const uint32_t nt = mg.ntiles1 - t0 < mg.G ? mg.ntiles1 - t0 : mg.G;
     const uint16_t *row0 = mg.off1 + (uint64_t)bb * mg.ntiles1 + t0;
+    uint32_t r0[1], r1[1];
+    load_rows<1>(row0, row0 + mg.ntiles1, nt, r0, r1);
     uint32_t t = gather_segments<KeyT, NT, ITEMS, 1, NT>(
-        reinterpret_cast<const KeyT *>(Jg.out1), row0, row0 + mg.ntiles1, nt,
+        reinterpret_cast<const KeyT *>(Jg.out1), r0, r1, nt,
         (uint64_t)t0 * mg.tcap1, mg.tcap1, s_g, s_gwave, &s_gtotal, dst, mn);
     if (t > (uint32_t)CAP) {
       if (threadIdx.x == 0) atomicOr(Jg.err, ERR_OVERFLOW);
@@ -1115,7 +1134,7 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
 DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t H, uint64_t *n_dev,
                            uint64_t *s_excl) {
   constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = opaque_tid();
   if ((tid >> 6) == 0) {
     if (tid == 0) lb_store(flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));
     uint64_t excl = 0;
@@ -1302,9 +1321,11 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
 
   KeyT keys[ITEMS];
   uint32_t mine;
+  uint32_t r0[TPT], r1[TPT];
+  load_rows<TPT>(row0, row0 + m.ngroups, m.ngroups, r0, r1);
   const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
-      in2, row0, row0 + m.ngroups, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg,
-      s_wave, &s_total, keys, mine);
+      in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
+      keys, mine);
   if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
   const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
 

@@ -701,6 +701,36 @@ DK_DEV uint32_t find_job_wg(const uint32_t *pre, uint32_t njobs, uint32_t item, 
   return __builtin_amdgcn_readfirstlane(*s_job);
 }
 
+// The descriptors and item prefixes of a multi-job launch are written before
+// it and only read by kernels: read them through the scalar data path (a
+// uniform index gives scalar loads into SGPRs).
+#define DK_CONSTANT __attribute__((address_space(4)))
+DK_DEV JobDesc job_desc(const BatchArgs &a, uint32_t j) {
+  static_assert(sizeof(JobDesc) % 4 == 0, "JobDesc is copied in 32-bit words");
+  const DK_CONSTANT uint32_t *src = (const DK_CONSTANT uint32_t *)(a.jobs + j);
+  JobDesc d;
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&d);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(JobDesc) / 4; ++i) dst[i] = src[i];
+  return d;
+}
+DK_DEV uint32_t pre_at(const uint32_t *pre, uint32_t j) {
+  return ((const DK_CONSTANT uint32_t *)pre)[j];
+}
+
+// The job owning the current item, with its item range [lo, hi): the items a
+// persistent workgroup takes mostly stay in one job, so it is looked up again
+// only when an item leaves the range.  Uniform; may end with a barrier.
+struct JobCursor {
+  uint32_t job = 0, lo = 0, hi = 0;
+};
+DK_DEV void cursor_seek(JobCursor &c, const uint32_t *pre, uint32_t njobs, uint32_t it, uint32_t *s_job) {
+  if (it >= c.lo && it < c.hi) return;
+  c.job = find_job_wg(pre, njobs, it, s_job);
+  c.lo = pre_at(pre, c.job);
+  c.hi = pre_at(pre, c.job + 1);
+}
+
 // ---------------------------------------------------------- L1 extraction --
 //
 // k_l1 tiles are record-aligned: a record's S window slots are split over
@@ -903,27 +933,27 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   __shared__ uint32_t s_wave[NT / 64], s_total;
   __shared__ uint32_t s_job;
   static_assert(NT >= 1024 && L1_MAXRT <= NT, "one histogram slot and record per thread");
-  const uint32_t nitems = ONE ? one.m.ntiles1 : a.pre1[a.njobs];
+  const uint32_t nitems = ONE ? one.m.ntiles1 : pre_at(a.pre1, a.njobs);
   const uint32_t G = gridDim.x, x = blockIdx.x & 7, nx = (G - x + 7) >> 3;
   const uint32_t per = nitems >> 3, rem = nitems & 7;
   const uint32_t lo = x * per + (x < rem ? x : rem);
   const uint32_t hi = lo + per + (x < rem ? 1u : 0u);
   uint32_t item = lo + (blockIdx.x >> 3);
-  uint32_t job = 0;
+  JobCursor cur;  // the job of the tile being loaded next
   if (item < hi) {
-    if (!ONE) job = find_job_wg(a.pre1, a.njobs, item, &s_job);
-    const JobDesc &Jf = ONE ? one : a.jobs[job];
-    l1_prefetch_raw<NT, RAWW>(Jf.rd, item - (ONE ? 0u : a.pre1[job]), s_raw);
+    if (!ONE) cursor_seek(cur, a.pre1, a.njobs, item, &s_job);
+    const ReadsDesc rdf = ONE ? one.rd : job_desc(a, cur.job).rd;
+    l1_prefetch_raw<NT, RAWW>(rdf, item - cur.lo, s_raw);
   }
   __builtin_amdgcn_s_waitcnt(VMCNT0);
   for (; item < hi; item += nx) {
-    const JobDesc &J = ONE ? one : a.jobs[job];
+    const JobDesc J = ONE ? one : job_desc(a, cur.job);
     const ReadsDesc rd = J.rd;
     const MsdArgs m = J.m;
     KeyT *out1 = reinterpret_cast<KeyT *>(J.out1);
     uint64_t *n_dev = J.nd;
     uint32_t *err = J.err;
-    const uint32_t tile = item - (ONE ? 0u : a.pre1[job]);
+    const uint32_t tile = item - cur.lo;
     const uint32_t tid = opaque_tid();
     __syncthreads();  // the previous tile is written out; this tile's records are in LDS
     s_h[tid] = 0;
@@ -937,9 +967,9 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
       // the records are decoded: the next tile's stream into the raw buffer
       const uint32_t nitem = item + nx;
       if (nitem < hi) {
-        if (!ONE) job = find_job_wg(a.pre1, a.njobs, nitem, &s_job);
-        const JobDesc &Jn = ONE ? one : a.jobs[job];
-        l1_prefetch_raw<NT, RAWW>(Jn.rd, nitem - (ONE ? 0u : a.pre1[job]), s_raw);
+        if (!ONE) cursor_seek(cur, a.pre1, a.njobs, nitem, &s_job);
+        const ReadsDesc rdn = ONE ? one.rd : job_desc(a, cur.job).rd;
+        l1_prefetch_raw<NT, RAWW>(rdn, nitem - cur.lo, s_raw);
       }
     });
     if (m.d1.num_pes <= 1) {  // first-level digit: the top bits alone
@@ -1006,7 +1036,7 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
   __shared__ uint32_t s_h[1024 + 1];
   __shared__ uint32_t s_wave[NT / 64], s_total, s_gwave[NT / 64], s_gtotal;
   __shared__ uint32_t s_job;
-  const uint32_t nitems = ONE ? one.m.nb1 * one.m.ngroups : a.pre2[a.njobs];
+  const uint32_t nitems = ONE ? one.m.nb1 * one.m.ngroups : pre_at(a.pre2, a.njobs);
   const uint32_t G = gridDim.x, x = blockIdx.x & 7, nx = (G - x + 7) >> 3;
   const uint32_t per = nitems >> 3, rem = nitems & 7;
   const uint32_t lo = x * per + (x < rem ? x : rem);
@@ -1014,11 +1044,11 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
   uint32_t item = lo + (blockIdx.x >> 3);
 
   // gather item `it` into dst (loads left in flight); returns the key count
-  auto gather_item = [&](uint32_t it, uint32_t &jb, KeyT (&dst)[ITEMS], uint32_t &mn) -> uint32_t {
-    if (!ONE) jb = find_job_wg(a.pre2, a.njobs, it, &s_job);
-    const JobDesc &Jg = ONE ? one : a.jobs[jb];
+  auto gather_item = [&](uint32_t it, JobCursor &jc, KeyT (&dst)[ITEMS], uint32_t &mn) -> uint32_t {
+    if (!ONE) cursor_seek(jc, a.pre2, a.njobs, it, &s_job);
+    const JobDesc Jg = ONE ? one : job_desc(a, jc.job);
     const MsdArgs mg = Jg.m;
-    const uint32_t wgi = it - (ONE ? 0u : a.pre2[jb]);
+    const uint32_t wgi = it - jc.lo;
     const uint32_t gg = wgi / mg.nb1, bb = wgi - gg * mg.nb1;
     const uint32_t t0 = gg * mg.G;
     const uint32_t nt = mg.ntiles1 - t0 < mg.G ? mg.ntiles1 - t0 : mg.G;
@@ -1036,14 +1066,15 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
   };
 
   KeyT v[ITEMS];
-  uint32_t job = 0, mine = 0, tot = 0;
-  if (item < hi) tot = gather_item(item, job, v, mine);
+  JobCursor cur;  // the job of the item being processed
+  uint32_t mine = 0, tot = 0;
+  if (item < hi) tot = gather_item(item, cur, v, mine);
   __builtin_amdgcn_s_waitcnt(VMCNT0);
   for (; item < hi; item += nx) {
-    const JobDesc &J = ONE ? one : a.jobs[job];
+    const JobDesc J = ONE ? one : job_desc(a, cur.job);
     const MsdArgs m = J.m;
     KeyT *out2 = reinterpret_cast<KeyT *>(J.out2);
-    const uint32_t wg = item - (ONE ? 0u : a.pre2[job]);
+    const uint32_t wg = item - cur.lo;
     const uint32_t g = wg / m.nb1, b = wg - g * m.nb1;
     const uint32_t tid = opaque_tid();
     s_h[tid] = 0;  // the previous item's split is done (its stage write-out follows a barrier)
@@ -1060,9 +1091,10 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     // the next item's keys stream in while this one is split; the gather's
     // barriers also order the ranking before the scan
     KeyT vn[ITEMS];
-    uint32_t jn = job, mn = 0, tn = 0;
+    JobCursor cn = cur;
+    uint32_t mn = 0, tn = 0;
     const uint32_t nitem = item + nx;
-    if (nitem < hi) tn = gather_item(nitem, jn, vn, mn);
+    if (nitem < hi) tn = gather_item(nitem, cn, vn, mn);
     else __syncthreads();
     scan_lds<NT>(s_h, m.nd2, s_wave, &s_total);
     uint16_t *off = m.off2 + (uint64_t)b * (m.nd2 + 1) * m.ngroups + g;
@@ -1089,7 +1121,7 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     }
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) v[u] = vn[u];
-    job = jn;
+    cur = cn;
     mine = mn;
     tot = tn;
   }
@@ -1296,7 +1328,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
   __syncthreads();
   const uint32_t item = __builtin_amdgcn_readfirstlane(s_sb);
   const uint32_t job = ONE ? 0u : find_job_wg(a.pre3, a.njobs, item, &s_job);
-  const JobDesc &J = ONE ? one : a.jobs[job];
+  const JobDesc J = ONE ? one : job_desc(a, job);
   if (tid == 0) {
     // distinct-key path: skip it once most earlier sub-buckets of the job
     // overflowed the table (read here, before the key loads are in flight)
@@ -1315,7 +1347,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
   for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
   for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
   __syncthreads();
-  const uint32_t sb = item - (ONE ? 0u : a.pre3[job]), nsb = m.nb1 * m.nd2;
+  const uint32_t sb = item - (ONE ? 0u : pre_at(a.pre3, job)), nsb = m.nb1 * m.nd2;
   const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
   const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
 

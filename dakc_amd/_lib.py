@@ -62,7 +62,7 @@ class Table(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_int32), ("algo", C.c_int32),
-                ("max_large", C.c_int32), ("reserved", C.c_int32)]
+                ("max_large", C.c_int32), ("batch_solo_items", C.c_int32)]
 
 
 class Stats(C.Structure):

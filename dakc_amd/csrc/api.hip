@@ -400,16 +400,24 @@ ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
 // ------------------------------------------------------ multi-job MSD --
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+// A job of a multi-job launch with this many final-level items (default)
+// fills the GPU a few hundred workgroups deep on its own: it gets a
+// final-level launch of its own, which saves every workgroup the lookup of
+// its job.
+constexpr uint32_t FIN_SOLO_ITEMS = 32768;
+uint32_t solo_items(const dakc_ctx *c) {
+  return c->opts.batch_solo_items > 0 ? (uint32_t)c->opts.batch_solo_items : FIN_SOLO_ITEMS;
+}
 constexpr size_t META_STRIDE = 32;  // per job: u64 n, u64 d, u32 err
 
 // Arena layout: all jobs' look-back flags first (one fill clears them), then
 // per job its reads copy (host input only), L1 tiles / table keys, L2 regions,
 // counts, offsets and owner-bin offsets.
-void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads) {
+void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads,
+                uint32_t solo) {
   const size_t nj = plans.size();
   b.plans = plans;
   b.kw = nj ? plans[0].kw : 1;
-  b.fin_nt = nj ? plans[0].fin_nt : 256;
   for (auto *v : {&b.off_reads, &b.off_out1, &b.off_out2, &b.off_counts, &b.off_off1, &b.off_off2,
                   &b.off_flags, &b.off_binoff})
     v->assign(nj, 0);
@@ -449,9 +457,22 @@ void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<b
     b.pre2[j + 1] = b.pre2[j] + p.nb1 * p.ngroups;
     b.pre3[j + 1] = b.pre3[j] + p.nsb;
   }
+  b.fin_runs.clear();
+  for (size_t j = 0; j < nj; ++j) {
+    const uint32_t fin = plans[j].fin_nt;
+    if (nj > 1 && plans[j].nsb >= solo) {
+      b.fin_runs.push_back({fin, b.pre3[j], b.pre3[j + 1], (int32_t)j});
+      continue;
+    }
+    if (b.fin_runs.empty() || b.fin_runs.back().solo >= 0 || b.fin_runs.back().fin_nt != fin ||
+        b.fin_runs.back().item1 != b.pre3[j])
+      b.fin_runs.push_back({fin, b.pre3[j], b.pre3[j], -1});
+    b.fin_runs.back().item1 = b.pre3[j + 1];
+  }
   b.arena_bytes = o;
   b.meta_bytes = al256(nj * META_STRIDE);
-  b.desc_bytes = al256(nj * sizeof(JobDesc)) + al256(3 * (nj + 1) * 4) + 256;
+  // descriptors, item prefixes, one ticket per final-level run
+  b.desc_bytes = al256(nj * sizeof(JobDesc)) + al256(3 * (nj + 1) * 4) + al256(b.fin_runs.size() * 4 + 4);
 }
 
 size_t msd_job_bytes(const Plan &pl, bool copy_reads) {
@@ -525,7 +546,7 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     }
     ws.hstage_cap = b.desc_bytes;
   }
-  if (nj) b.one = desc[0];
+  b.descs = desc;
   uint8_t *host = ws.hstage;
   std::memset(host, 0, b.desc_bytes);
   std::memcpy(host, desc.data(), nj * sizeof(JobDesc));
@@ -552,6 +573,7 @@ static BatchArgs batch_args(const JobWs &ws, const MsdBatch &b) {
   a.pre3 = pre + 2 * (nj + 1);
   a.njobs = (uint32_t)nj;
   a.ticket = pre + 3 * (nj + 1);
+  a.item0 = 0;
   return a;
 }
 
@@ -563,14 +585,22 @@ void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
   const BatchArgs a = batch_args(ws, b);
   launch_fill_u64((uint64_t *)ws.meta.p, 0, b.meta_bytes / 8, st);
   launch_fill_u64((uint64_t *)ws.arena.p + b.flags_off / 8, 0, b.flags_words, st);
-  launch_fill_u32(a.ticket, 0, 1, st);
+  launch_fill_u32(a.ticket, 0, std::max<size_t>(b.fin_runs.size(), 1), st);
   if (events) hipEventRecord(ws.ev[0], st);
-  const JobDesc *one = nj == 1 ? &b.one : nullptr;
-  launch_msd_batch(b.kw, b.fin_nt, a, one, b.pre1[nj], 0, 0, st);
+  const JobDesc *one = nj == 1 ? &b.descs[0] : nullptr;
+  const uint32_t fin0 = b.fin_runs.empty() ? 256u : b.fin_runs[0].fin_nt;
+  launch_msd_batch(b.kw, fin0, a, one, b.pre1[nj], 0, 0, st);
   if (events) hipEventRecord(ws.ev[1], st);
-  launch_msd_batch(b.kw, b.fin_nt, a, one, 0, b.pre2[nj], 0, st);
+  launch_msd_batch(b.kw, fin0, a, one, 0, b.pre2[nj], 0, st);
   if (events) hipEventRecord(ws.ev[2], st);
-  launch_msd_batch(b.kw, b.fin_nt, a, one, 0, 0, b.pre3[nj], st);
+  for (size_t r = 0; r < b.fin_runs.size(); ++r) {
+    const MsdBatch::FinRun &fr = b.fin_runs[r];
+    BatchArgs ar = a;
+    ar.ticket = a.ticket + r;
+    ar.item0 = fr.solo >= 0 ? 0u : fr.item0;  // a solo launch numbers its job's items from 0
+    launch_msd_batch(b.kw, fr.fin_nt, ar, fr.solo >= 0 ? &b.descs[fr.solo] : one, 0, 0,
+                     fr.item1 - fr.item0, st);
+  }
   for (size_t j = 0; j < nj; ++j)
     if (b.plans[j].owner_digit)
       launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.locs[j].binoff,

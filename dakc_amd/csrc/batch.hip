@@ -74,8 +74,9 @@ static int table_to_host(const MsdBatch &b, size_t j, uint64_t n, uint64_t d, da
   return DAKC_OK;
 }
 
-// Count the jobs listed in `idx` (all MSD-planned with the same key width and
-// final-level size) in one multi-job launch on ws; overflowed jobs go to `redo`.
+// Count the jobs listed in `idx` (all MSD-planned with the same key width,
+// ordered by final-level size) in one multi-job launch on ws; overflowed jobs
+// go to `redo`.
 static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std::vector<Plan> &plans,
                          const std::vector<size_t> &idx, dakc_table *outs, std::vector<int> &status,
                          std::vector<size_t> &redo) {
@@ -88,7 +89,7 @@ static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std
     copy.push_back(!jobs[i].reads_on_device);
     js.push_back(&jobs[i]);
   }
-  msd_layout(b, ps, copy);
+  msd_layout(b, ps, copy, solo_items(c));
   int s = msd_stage(ws, b, js);
   if (s) return s;
   msd_enqueue(ws, b, false);
@@ -106,7 +107,6 @@ static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std
     if (!st) st = table_to_host(b, j, n, d, &outs[idx[j]]);
     status[idx[j]] = st;
   }
-  (void)c;
   return DAKC_OK;
 }
 
@@ -115,13 +115,13 @@ int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *out
   std::vector<Plan> plans(njobs);
   std::vector<int> status(njobs, DAKC_OK);
   std::vector<size_t> plain;  // counted one at a time (LSD path, or no MSD geometry)
-  std::map<std::pair<int, uint32_t>, std::vector<size_t>> groups;
+  std::map<int, std::vector<size_t>> groups;  // by key width
   for (size_t i = 0; i < njobs; ++i) {
     std::memset(&outs[i], 0, sizeof(outs[i]));
     status[i] = make_plan(&jobs[i], &plans[i]);
     if (status[i]) continue;
     apply_tuning(c, &plans[i], c->opts.algo);
-    if (plans[i].msd) groups[{plans[i].kw, plans[i].fin_nt}].push_back(i);
+    if (plans[i].msd) groups[plans[i].kw].push_back(i);
     else plain.push_back(i);
   }
   const size_t budget = hbm_budget(c);
@@ -136,6 +136,12 @@ int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *out
       const size_t jb = k < ids.size() ? msd_job_bytes(plans[ids[k]], !jobs[ids[k]].reads_on_device) : 0;
       if (k == ids.size() || (!chunk.empty() && bytes + jb > budget)) {
         if (!chunk.empty()) {
+          // small jobs of one final-level size are consecutive (one shared
+          // final launch), large ones follow
+          std::stable_sort(chunk.begin(), chunk.end(), [&](size_t a, size_t b) {
+            return plans[a].fin_nt != plans[b].fin_nt ? plans[a].fin_nt < plans[b].fin_nt
+                                                      : plans[a].nsb < plans[b].nsb;
+          });
           int s = run_msd_group(c, c->ws, jobs, plans, chunk, outs, status, redo);
           if (s)
             for (size_t i : chunk) status[i] = s;
@@ -194,7 +200,7 @@ extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njo
   b->ctx = ctx;
   b->jobs.assign(jobs, jobs + njobs);
   b->plans.resize(njobs);
-  size_t lead = njobs;  // the largest MSD job: its key width and final-level size rule the launch
+  size_t lead = njobs;  // the largest MSD job: its key width rules the launch
   for (size_t i = 0; i < njobs; ++i) {
     int s = make_plan(&jobs[i], &b->plans[i]);
     if (s) {
@@ -208,18 +214,17 @@ extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njo
     if (p.msd && (lead == njobs || p.nmax > b->plans[lead].nmax)) lead = i;
   }
   const int kw = lead < njobs ? b->plans[lead].kw : -1;
-  const uint32_t fin = lead < njobs ? b->plans[lead].fin_nt : 0;
   for (size_t i = 0; i < njobs; ++i) {
-    Plan &p = b->plans[i];
-    if (p.msd && p.kw == kw && p.fin_nt != fin) {
-      // one captured launch takes one final-level size: re-plan this job to
-      // the group's (the workgroup size changes the time, never the counts)
-      p.tune_fin_nt = fin;
-      plan_choose_path(&p, DAKC_ALGO_MSD);
-    }
-    if (p.msd && p.kw == kw && p.fin_nt == fin) b->msd_idx.push_back(i);
+    const Plan &p = b->plans[i];
+    if (p.msd && p.kw == kw) b->msd_idx.push_back(i);
     else b->plain_idx.push_back(i);
   }
+  // every job keeps its tuned final-level size; small jobs of one size are
+  // consecutive (one shared final launch), large ones follow
+  std::stable_sort(b->msd_idx.begin(), b->msd_idx.end(), [&](size_t x, size_t y) {
+    const Plan &p = b->plans[x], &q = b->plans[y];
+    return p.fin_nt != q.fin_nt ? p.fin_nt < q.fin_nt : p.nsb < q.nsb;
+  });
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       b->ws.init(b->stream) || hipEventCreate(&b->e0) != hipSuccess ||
       hipEventCreate(&b->e1) != hipSuccess) {
@@ -234,7 +239,7 @@ extern "C" int dakc_batch_create(dakc_ctx *ctx, const dakc_job *jobs, size_t njo
     copy.push_back(!jobs[i].reads_on_device);
     js.push_back(&b->jobs[i]);
   }
-  msd_layout(b->layout, ps, copy);
+  msd_layout(b->layout, ps, copy, solo_items(ctx));
   int s = msd_stage(b->ws, b->layout, js);  // host reads are copied once, here
   if (s) {
     dakc_batch_destroy(b);

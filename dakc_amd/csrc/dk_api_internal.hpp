@@ -85,7 +85,16 @@ struct TableLoc {
 // argument block (JobDesc table, work-item prefixes, ticket).
 struct MsdBatch {
   int kw = 1;
-  uint32_t fin_nt = 256;
+  // Final-level launches.  A job with at least FIN_SOLO_ITEMS sub-buckets has
+  // a launch of its own (solo = its index: its descriptor is a kernel
+  // argument, as in a one-job count); consecutive smaller jobs with one
+  // workgroup size share a work-queue launch over their items [item0, item1)
+  // (solo = -1), where each workgroup looks its job up.
+  struct FinRun {
+    uint32_t fin_nt, item0, item1;
+    int32_t solo;
+  };
+  std::vector<FinRun> fin_runs;
   std::vector<Plan> plans;
   std::vector<size_t> off_reads, off_out1, off_out2, off_counts, off_off1, off_off2, off_flags,
       off_binoff;
@@ -93,7 +102,7 @@ struct MsdBatch {
   size_t arena_bytes = 0, flags_off = 0, flags_words = 0;
   size_t meta_bytes = 0, desc_bytes = 0;
   std::vector<TableLoc> locs;
-  JobDesc one;  // the descriptor of job 0, a kernel argument when the batch holds one job
+  std::vector<JobDesc> descs;  // host copies of the job descriptors
 };
 
 // Device workspace of one in-flight job (one per served slot).
@@ -132,7 +141,11 @@ struct JobWs {
 int make_plan(const dakc_job *job, Plan *pl);
 // multi-job MSD: lay out, stage (allocate, upload arguments, copy host reads),
 // enqueue (fills + kernels; capturable in a hipGraph), collect per-job status
-void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads);
+// solo: final-level items from which a job of a multi-job layout gets a
+// final-level launch of its own
+void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<bool> &copy_reads,
+                uint32_t solo = 0xFFFFFFFFu);
+uint32_t solo_items(const dakc_ctx *c);
 int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs);
 void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events);
 int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, uint64_t *d);

@@ -142,9 +142,12 @@ struct BatchArgs {
   const JobDesc *jobs;
   const uint32_t *pre1, *pre2, *pre3;  // njobs + 1 prefixes of L1 / L2 / final items
   uint32_t njobs;
-  uint32_t *ticket;                    // final-level ticket
+  uint32_t *ticket;                    // final-level ticket of this launch
+  uint32_t item0;                      // first final-level item of this launch
 };
-// launch all three MSD levels for every job of `a` (kw and fin_nt shared by the jobs)
+// launch the MSD levels for every job of `a` (n1 L1 items, n2 L2 items, n3
+// final items from a.item0 on; a zero count skips the level).  kw is shared
+// by all jobs, fin_nt by the jobs whose final items the launch covers.
 // one: the descriptor of a single-job launch (passed as a kernel argument), or
 // null when the launch holds several jobs (read from a.jobs)
 void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc *one, uint32_t n1,

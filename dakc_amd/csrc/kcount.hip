@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
   __syncthreads();
-  const uint32_t item = __builtin_amdgcn_readfirstlane(s_sb);
+  const uint32_t item = a.item0 + __builtin_amdgcn_readfirstlane(s_sb);
   const uint32_t job = ONE ? 0u : find_job_wg(a.pre3, a.njobs, item, &s_job);
   const JobDesc J = ONE ? one : job_desc(a, job);
   if (tid == 0) {

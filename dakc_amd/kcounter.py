@@ -79,12 +79,14 @@ class KmerCounter:
 
     def __init__(self, k: int = 31, read_len: int = 150, *, canonical: bool = False,
                  num_pes: int = 1, owner_bin: int = -1, device: int = 0,
-                 hbm_budget: int = 0, max_inflight: int = 0, algo: int = 0, max_large: int = 0):
+                 hbm_budget: int = 0, max_inflight: int = 0, algo: int = 0, max_large: int = 0,
+                 batch_solo_items: int = 0):
         self.k, self.read_len, self.canonical = k, read_len, canonical
         self.num_pes, self.owner_bin = num_pes, owner_bin
         self._libh = _lib.load()
         self._ctx = C.c_void_p()
-        opts = Opts(hbm_budget=hbm_budget, max_inflight=max_inflight, algo=algo, max_large=max_large)
+        opts = Opts(hbm_budget=hbm_budget, max_inflight=max_inflight, algo=algo, max_large=max_large,
+                    batch_solo_items=batch_solo_items)
         check(self._libh.dakc_ctx_create(device, C.byref(opts), C.byref(self._ctx)),
               "dakc_ctx_create")
 

@@ -127,18 +127,26 @@ def test_autotune_keeps_counts_exact(tmp_path):
         assert np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
 
 
-def test_captured_batch_takes_every_job_across_tuned_classes(tmp_path):
-    """Size classes tuned to different final-level sizes (and one to the LSD
-    path) still go into the one captured launch, with exact counts."""
-    path = tmp_path / "tune.json"
+def mixed_tuning(path):
+    """A tuning file whose size classes ask for every final-level size (and
+    one for the LSD path); read_sets(11, 8) falls in classes 17, 19, 20, 22."""
     rows = []
     for cls in range(12, 28):  # classes: log2 of the k-mer slots
-        fin, algo = (256, "msd") if cls % 2 else (1024, "msd")
+        fin, algo = {17: 256, 19: 512, 22: 1024}.get(cls, (256, 512, 1024)[cls % 3]), "msd"
         if cls == 20:
             algo = "lsd"
         rows.append(f'  {{"key_words": 1, "size_class": {cls}, "fin_nt": {fin}, "fill_pct": 60, '
                     f'"algo": "{algo}", "ms": 1.0}}')
     path.write_text('{\n "entries": [\n' + ",\n".join(rows) + "\n ]\n}\n")
+
+
+def test_captured_batch_takes_every_job_across_tuned_classes(tmp_path):
+    """Size classes tuned to different final-level sizes (and one to the LSD
+    path) still go into the one captured launch, each job keeping its own
+    final-level size (one final launch per size), with exact counts; the
+    uncaptured batch groups them the same way."""
+    path = tmp_path / "tune.json"
+    mixed_tuning(path)
     sets = read_sets(11, 8)
     want = [oracle.count(s, 250, 31) for s in sets]
     with KmerCounter(k=31, read_len=250) as kc:
@@ -150,7 +158,31 @@ def test_captured_batch_takes_every_job_across_tuned_classes(tmp_path):
             got = b.tables()
         for (keys, counts, n, _), t in zip(want, got):
             assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+        for (keys, counts, n, _), t in zip(want, kc.count_batch(sets)):
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
 
+
+@pytest.mark.parametrize("solo", [1, 512])
+def test_batch_solo_final_launches_match_oracle(tmp_path, solo):
+    """Jobs with at least `solo` final-level items get a final launch of their
+    own (solo=1: every job; 512: the larger sets, the rest share the
+    work-queue launch), beside jobs of other final-level sizes: exact counts,
+    captured and uncaptured."""
+    path = tmp_path / "tune.json"
+    mixed_tuning(path)
+    sets = read_sets(11, 8)
+    want = [oracle.count(s, 250, 31) for s in sets]
+    with KmerCounter(k=31, read_len=250, batch_solo_items=solo) as kc:
+        kc.tuning_load(path)
+        with kc.batch_create(sets) as b:
+            assert b.info()[0] == len(sets)
+            for _ in range(2):
+                b.run()
+                got = b.tables()
+                for (keys, counts, n, _), t in zip(want, got):
+                    assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+        for (keys, counts, n, _), t in zip(want, kc.count_batch(sets)):
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
 
 def test_served_device_tables_match_oracle():
     """Served jobs that keep their table in device memory: each table equals

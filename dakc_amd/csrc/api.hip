@@ -105,6 +105,13 @@ void *DevArena::alloc_top(size_t bytes) {
   return nullptr;
 }
 
+size_t DevArena::largest_free() {
+  std::lock_guard<std::mutex> g(mu);
+  size_t m = 0;
+  for (const auto &r : free_ranges) m = std::max(m, r.second);
+  return m;
+}
+
 void DevArena::release(void *p, size_t bytes) {
   if (!p) return;
   bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;

@@ -36,6 +36,7 @@ struct DevArena {
   void *alloc(size_t bytes);
   void *alloc_top(size_t bytes);
   void release(void *p, size_t bytes);
+  size_t largest_free();  // bytes of the largest free range
 };
 
 // A grow-only device buffer: from hipMalloc, or (served slots) from an arena.

@@ -36,7 +36,17 @@ struct Slot {
   hipEvent_t done_ev = nullptr;
   uint64_t ticket = 0;  // 0 = free
   bool large = false;   // the job fills the GPU by itself
+  bool msd = false;     // on the MSD path: ws.ev[2] marks its second level done
 };
+
+// A large job holds the GPU to the end, or, under a backlog (pipelined), only
+// until its final level: the next large job's first levels then fill the GPU
+// beside the final level instead of after it.  That raises throughput but
+// stretches the earlier job's final level, so it is kept for backlogs.
+static bool holds_large(Slot &sl, bool pipelined) {
+  if (!sl.ticket || !sl.large) return false;
+  return !pipelined || !sl.msd || hipEventQuery(sl.ws.ev[2]) == hipErrorNotReady;
+}
 
 // A job fills the GPU by itself when its first level alone is 16 waves of
 // workgroups over all CUs; two such jobs side by side only share the GPU and
@@ -44,6 +54,12 @@ struct Slot {
 // fill the other slots.
 static bool is_large(const Plan &p, int num_cus) {
   return p.nmax >= (uint64_t)num_cus * 16 * (uint64_t)msd_tile(p.kw);
+}
+
+// Most pool bytes a job can hold once done: its workspace while it runs,
+// then its table (every k-mer distinct at worst) until the client claims it.
+static size_t job_pool_bound(const Plan &p) {
+  return job_device_bytes(p) + (size_t)p.nmax * ((size_t)p.kw * 8 + 4) + 1024;
 }
 
 struct ServeState {
@@ -109,13 +125,19 @@ static void release_except(DevArena &a, const Ranges &job, const std::pair<void 
   }
 }
 
+// Tables up to this size move to the top of the pool (a copy of well under a
+// millisecond), so the bottom stays contiguous for job buffers; larger ones
+// stay where the count wrote them: moving them would cost a few percent of
+// their count in HBM traffic.
+constexpr size_t TABLE_MOVE_MAX = (size_t)32 << 20;
+
 // The slot's count has finished: check it (rerun an LDS overflow on the LSD
-// path), then move the table to a range at the top of the pool with a device
-// copy on the slot's stream; the job's buffers go back once the copy is done,
-// and the slot takes its next job at once (its stream orders that job after
-// the copy).  When the top has no room the table stays where the count left
-// it: the two ranges of the job's buffers that hold its keys and counts
-// become the table's and the rest goes back to the pool.
+// path), then move a small table to a range at the top of the pool with a
+// device copy on the slot's stream; the job's buffers go back once the copy
+// is done, and the slot takes its next job at once (its stream orders that
+// job after the copy).  A large table, or one the top has no room for, stays
+// where the count left it: the two ranges of the job's buffers that hold its
+// keys and counts become the table's and the rest goes back to the pool.
 static void retire(ServeState &s, Slot &sl) {
   const uint64_t id = sl.ticket;
   Ticket &t = *s.tickets[id];
@@ -125,6 +147,7 @@ static void retire(ServeState &s, Slot &sl) {
     // a sub-bucket did not fit in LDS: count this job again on the LSD path
     plan_choose_path(&t.plan, DAKC_ALGO_LSD);
     ++s.stats.lsd_reruns;
+    sl.msd = false;
     drop_job_buffers(sl);
     st = sl.ws.reserve(t.plan, !t.job.reads_on_device);
     if (!st)
@@ -169,7 +192,7 @@ static void retire(ServeState &s, Slot &sl) {
   o.priv = h;
   const size_t kb = (size_t)d * t.plan.kw * 8, cb = (size_t)d * 4;
   const size_t tb = al256(kb) + al256(cb);
-  void *dst = d ? s.arena->alloc_top(tb) : nullptr;
+  void *dst = d && tb <= TABLE_MOVE_MAX ? s.arena->alloc_top(tb) : nullptr;
   hipEvent_t ev = nullptr;
   if (dst) {
     if (!s.spare_events.empty()) {
@@ -269,14 +292,18 @@ static void scheduler(ServeState *s) {
     // admit queued jobs in FIFO order into free slots while their buffers fit
     // the pool; a large job waits while another large one runs, and smaller
     // jobs behind it may go first meanwhile (a job that does not fit the
-    // pool keeps its place and everything behind it waits)
-    // under a backlog longer than the slots, throughput comes first: large
-    // jobs may then share the GPU in every slot
-    const int nslots = (int)s->slots.size();
-    const int max_large = (int)s->queue.size() > nslots ? nslots
-                          : s->ctx->opts.max_large > 0 ? s->ctx->opts.max_large : 1;
+    // pool keeps its place and everything behind it waits).  Large jobs that
+    // shared the GPU would finish together, later than one at a time would
+    // finish the first: the latency tail of a burst.
+    // A job goes before queued jobs only if the pool keeps room for the
+    // largest of their workspaces beside the overtaker's workspace and table:
+    // a client that claims tables in submission order then never waits on a
+    // job that unclaimed tables of later jobs keep out of the pool.
+    const int max_large = s->ctx->opts.max_large > 0 ? s->ctx->opts.max_large : 1;
+    size_t passed = 0;  // largest workspace among the queued jobs passed over
     int large_running = 0;
-    for (auto &sl : s->slots) large_running += sl->ticket && sl->large;
+    const bool pipelined = s->queue.size() > 4 * s->slots.size();
+    for (auto &sl : s->slots) large_running += holds_large(*sl, pipelined);
     for (auto qi = s->queue.begin(); qi != s->queue.end();) {
       Slot *slp = nullptr;
       for (auto &sl : s->slots)
@@ -289,15 +316,17 @@ static void scheduler(ServeState *s) {
       const uint64_t id = *qi;
       Ticket &t = *s->tickets[id];
       const bool large = is_large(t.plan, s->ctx->num_cus);
-      if (large && large_running >= max_large) {
+      if ((large && large_running >= max_large) ||
+          (passed && s->arena->largest_free() < passed + job_pool_bound(t.plan))) {
+        passed = std::max(passed, job_device_bytes(t.plan));
         ++qi;
         continue;
       }
       const auto h0 = std::chrono::steady_clock::now();
       int st = sl.ws.reserve(t.plan, !t.job.reads_on_device);
-      if (!st)
+      if (!st)  // with stage events: ws.ev[2] tells when its final level starts
         st = enqueue_count(sl.ws, t.plan, &t.job, (const uint8_t *)t.job.reads, s->ctx->num_cus,
-                           false);
+                           true);
       if (!st && hipEventRecord(sl.done_ev, sl.stream) != hipSuccess) st = DAKC_E_DEVICE;
       s->stats.admit_host_ms +=
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
@@ -324,6 +353,7 @@ static void scheduler(ServeState *s) {
       ++s->stats.admitted;
       sl.ticket = id;
       sl.large = large;
+      sl.msd = t.plan.msd;
       large_running += large;
       any_running = true;
       progressed = true;

@@ -100,8 +100,7 @@ typedef struct dakc_opts {
   int32_t max_inflight; /* served mode: concurrent jobs on the device (0 = 4) */
   int32_t algo;         /* DAKC_ALGO_* */
   int32_t max_large;    /* served mode: jobs that fill the GPU by themselves (first level
-                         * >= 16 waves of workgroups) in flight at once while the queue
-                         * is no longer than max_inflight (0 = 1); beyond, any number */
+                         * >= 16 waves of workgroups) in flight at once (0 = 1) */
   int32_t batch_solo_items; /* batched: a job with at least this many final-level work items
                              * (sub-buckets) gets a final-level launch of its own instead of
                              * sharing the work-queue launch of the smaller jobs (0 = 32768) */

@@ -33,8 +33,9 @@ WORKLOADS = {
     # genome-like single job: 1e8 bp genome, 30x of 150 bp reads, 0.5% errors, k=31
     "genome": dict(seed=2025, num_reads=20_000_000, read_len=150, genome_len=100_000_000,
                    sub_rate=0.005, k=31),
-    # plain single job: 1M i.i.d. reads x 150, k=31
-    "iid": dict(seed=2024, num_reads=1_000_000, read_len=150, genome_len=0, sub_rate=0.0, k=31),
+    # plain single job: 1M i.i.d. reads x 150, canonical k=31
+    "iid": dict(seed=2024, num_reads=1_000_000, read_len=150, genome_len=0, sub_rate=0.0, k=31,
+                canonical=True),
 }
 
 
@@ -59,11 +60,12 @@ def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
     p = _lib.gen_params(w["seed"], sample_reads, w["read_len"], genome_len=w["genome_len"],
                         sub_rate=w["sub_rate"], min_len=w.get("min_len", 0))
     reads = _lib.generate_reads(p, threads=threads)
-    oracle.count_mt(reads, w["read_len"], w["k"], threads)  # warm-up
+    can = w.get("canonical", False)
+    oracle.count_mt(reads, w["read_len"], w["k"], threads, canonical=can)  # warm-up
     times = []
     n = 0
     for _ in range(3):
-        n, d, cs, el = oracle.count_mt(reads, w["read_len"], w["k"], threads)
+        n, d, cs, el = oracle.count_mt(reads, w["read_len"], w["k"], threads, canonical=can)
         times.append(el)
     t = float(np.median(times))
     return {"value": n / t, "unit": "k-mers/s", "cores": threads, "kind": "port",
@@ -331,7 +333,7 @@ def main() -> int:
         return bench_serve(a)
 
     w = WORKLOADS[a.workload]
-    kc = KmerCounter(k=w["k"], read_len=w["read_len"],
+    kc = KmerCounter(k=w["k"], read_len=w["read_len"], canonical=w.get("canonical", False),
                      algo=_lib.ALGO_LSD if a.algo == "lsd" else _lib.ALGO_AUTO)
     p = _lib.gen_params(w["seed"], w["num_reads"], w["read_len"], genome_len=w["genome_len"],
                         sub_rate=w["sub_rate"])
@@ -418,7 +420,8 @@ def main() -> int:
         "data": "synthetic (seeded counter-based generator, generated on the device)",
         "config": {"workload": f"{a.workload}: {w['num_reads']} reads x {w['read_len']} bp, "
                                f"genome {w['genome_len']} bp, {w['sub_rate'] * 100:.1f}% subs, "
-                               f"k={w['k']}", "k": w["k"], "kmers_per_step": n_kmers,
+                               f"k={w['k']}" + (" canonical" if w.get("canonical") else ""),
+                   "k": w["k"], "kmers_per_step": n_kmers,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,

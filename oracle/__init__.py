@@ -46,8 +46,8 @@ def load():
                               C.c_int32, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64),
                               P(C.c_uint64), C.c_void_p]
     lib.dko_count_mt.restype = C.c_int
-    lib.dko_count_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, P(C.c_uint64),
-                                 P(C.c_uint64), P(C.c_uint64), P(C.c_double)]
+    lib.dko_count_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 P(C.c_uint64), P(C.c_uint64), P(C.c_uint64), P(C.c_double)]
     lib.dko_table_checksum.restype = C.c_uint64
     lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     lib.dko_free.restype = None
@@ -113,12 +113,13 @@ def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1):
     return kk, cc, int(n.value), offs
 
 
-def count_mt(reads, read_len, k, threads):
+def count_mt(reads, read_len, k, threads, canonical=False):
     """DAKC's P-PE shape on `threads` CPU threads; returns (n, d, checksum, seconds)."""
     lib = load()
     b = _buf(reads)
     n, d, cs, el = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
-    st = lib.dko_count_mt(b.ctypes.data if b.size else None, b.size, read_len, k, threads,
+    st = lib.dko_count_mt(b.ctypes.data if b.size else None, b.size, read_len, k,
+                          1 if canonical else 0, threads,
                           C.byref(n), C.byref(d), C.byref(cs), C.byref(el))
     if st:
         raise OracleError(st)

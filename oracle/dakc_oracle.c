@@ -333,7 +333,7 @@ int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonic
 /* -------------------------------------------------- threaded P-PE shape --- */
 
 typedef struct {
-    int tid, T, read_len, k;
+    int tid, T, read_len, k, canonical;
     const char *reads;
     uint64_t r0, r1;
     uint64_t *local;      /* this worker's k-mers (phase 1) */
@@ -350,7 +350,7 @@ static void *mt_worker(void *p) {
     mt_arg *a = (mt_arg *)p;
     int T = a->T;
     /* Phase 1: parse this worker's block of records (fqreader.cpp:43-52) */
-    emit_ctx c = {a->k, 0, 1, 1, -1, a->local, a->nlocal, 0};
+    emit_ctx c = {a->k, a->canonical, 1, 1, -1, a->local, a->nlocal, 0};
     extract_range(a->reads, a->read_len, a->r0, a->r1, &c);
     a->nlocal = c.n;
     uint64_t *mycnt = a->cnt + (size_t)a->tid * T;
@@ -393,7 +393,7 @@ static double now_s(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int threads,
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, int threads,
                  uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s) {
     if (threads < 1 || threads > 1024 || k < 1 || k > 32) return DKO_E_INVAL;
     int64_t nrec = count_records(reads, nbytes, read_len);
@@ -421,7 +421,8 @@ int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int thre
     double t0 = now_s();
     for (int t = 0; t < T; t++) {
         mt_arg *a = &args[t];
-        a->tid = t; a->T = T; a->read_len = read_len; a->k = k; a->reads = reads;
+        a->tid = t; a->T = T; a->read_len = read_len; a->k = k; a->canonical = canonical != 0;
+        a->reads = reads;
         a->r0 = (uint64_t)nrec * t / T;
         a->r1 = (uint64_t)nrec * (t + 1) / T;
         a->local = local + a->r0 * per_read;

@@ -76,7 +76,7 @@ int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonic
  * multiset (dko_table_checksum) so the result can be compared with any table.
  * *elapsed_s is the wall time of the counting phases (file read excluded, as
  * kcounter.cpp:262). */
-int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int threads,
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, int threads,
                  uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s);
 
 /* order-independent checksum of a table: sum over entries of mix(key words, count) */

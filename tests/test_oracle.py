@@ -178,8 +178,9 @@ def test_format_errors_and_empty():
 
 def test_threaded_shape_matches_single_pe():
     data = random_reads(3, 3000, 150, alphabet="ACGTACGTACGTN", mpad=False)
-    keys, counts, n, _ = oracle.count(data, 150, 31)
-    for T in (1, 3, 8):
-        n2, d2, cs, el = oracle.count_mt(data, 150, 31, T)
-        assert (n2, d2) == (n, len(counts))
-        assert cs == oracle.checksum(keys, counts)
+    for canonical in (False, True):
+        keys, counts, n, _ = oracle.count(data, 150, 31, canonical=canonical)
+        for T in (1, 3, 8):
+            n2, d2, cs, el = oracle.count_mt(data, 150, 31, T, canonical=canonical)
+            assert (n2, d2) == (n, len(counts))
+            assert cs == oracle.checksum(keys, counts)

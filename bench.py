@@ -74,6 +74,46 @@ def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
                       f"{t:.3f} s each"}
 
 
+def post_count(kc, t, kb: int) -> dict:
+    """After the count, on its device table `t` (once each, after one untimed
+    call): the 64-bucket count histogram, point queries (5M keys taken from
+    the table, 5M random keys) and the solid-k-mer filter at the histogram's
+    minimum bucket, which changes the table and so runs last."""
+    from dakc_amd.kcounter import _ptr
+    d = int(t.n)
+    kc.table_histogram(t, 64)
+    t0 = time.perf_counter()
+    hist, m = kc.table_histogram(t, 64)
+    h_s = time.perf_counter() - t0
+    rng = np.random.default_rng(11)
+    blocks, per = 100, 50_000
+    present = [kc.d2h(_ptr(t.keys) + int(s) * kb, per * kb, np.uint64)
+               for s in np.linspace(0, max(d - per, 0), blocks).astype(np.int64)] if d >= per else []
+    kw = kb // 8
+    q = np.concatenate([np.concatenate(present).reshape(-1, kw) if present else np.zeros((0, kw), np.uint64),
+                        rng.integers(0, 2**62, size=(blocks * per, kw), dtype=np.uint64)])
+    dq, do = kc.device_alloc(q.nbytes), kc.device_alloc(q.shape[0] * 4)
+    kc.h2d(dq, q)
+    kc.table_query(t, dev_queries=dq, nq=q.shape[0], dev_out=do)
+    t0 = time.perf_counter()
+    kc.table_query(t, dev_queries=dq, nq=q.shape[0], dev_out=do)
+    q_s = time.perf_counter() - t0
+    hits = int((kc.d2h(do, q.shape[0] * 4, np.uint32) > 0).sum())
+    kc.device_free(dq)
+    kc.device_free(do)
+    t0 = time.perf_counter()
+    kc.table_filter(t, m)
+    f_s = time.perf_counter() - t0
+    kept = int(t.n)
+    return {"histogram": {"buckets": 64, "ms": h_s * 1e3, "alg_bytes": d * 4,
+                          "GBps": d * 4 / h_s / 1e9, "min_bucket": m},
+            "query": {"nq": int(q.shape[0]), "hits": hits, "ms": q_s * 1e3,
+                      "queries_per_s": q.shape[0] / q_s},
+            "filter": {"min_count": m, "kept": kept, "ms": f_s * 1e3,
+                       "alg_bytes": (d + kept) * (kb + 4), "GBps": (d + kept) * (kb + 4) / f_s / 1e9},
+            "timing": "wall clock around each synchronous call (one run, not in the timed counts)"}
+
+
 def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
@@ -315,6 +355,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the after-the-count histogram / query / filter measurement")
     ap.add_argument("--algo", default="auto", choices=["auto", "lsd"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also copy the read buffer device-to-device once (a plain copy kernel "
@@ -396,6 +438,7 @@ def main() -> int:
     dom_ms, dom_bytes = kernels[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     traffic = measured_traffic(a.workload, dom)
+    post = post_count(kc, t, kb) if not a.no_post else None
     stages = {
         "path": "msd" if path == 2 else "lsd", "msd_reruns": reruns, "distinct": d,
         "kernels": {name: {"avg_ms": t_ms, "alg_bytes": nb_,
@@ -427,6 +470,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
+        "post_count": post,
     }
     kc.device_free(dreads)
     kc.close()

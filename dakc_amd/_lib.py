@@ -31,7 +31,7 @@ EXPORTS = [
     "dakc_generate_reads_device", "dakc_device_alloc", "dakc_device_free", "dakc_memcpy_h2d",
     "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
-    "dakc_tuning_load",
+    "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
 ]
 
 
@@ -139,6 +139,9 @@ def load() -> C.CDLL:
         "dakc_autotune": (C.c_int, [V, P(Job), C.c_int, P(C.c_double)]),
         "dakc_tuning_save": (C.c_int, [V, C.c_char_p]),
         "dakc_tuning_load": (C.c_int, [V, C.c_char_p]),
+        "dakc_table_histogram": (C.c_int, [V, P(Table), C.c_uint32, V, P(C.c_uint32)]),
+        "dakc_table_filter": (C.c_int, [V, P(Table), C.c_uint32]),
+        "dakc_table_query": (C.c_int, [V, P(Table), V, C.c_uint64, C.c_int, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -757,6 +757,8 @@ extern "C" int dakc_ctx_destroy(dakc_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->ws.~JobWs();
   new (&c->ws) JobWs();
+  c->post_a.release();
+  c->post_b.release();
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return DAKC_OK;

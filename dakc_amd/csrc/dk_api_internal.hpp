@@ -180,6 +180,7 @@ struct dakc_ctx {
   std::mutex serve_init_mu;
   std::mutex tune_mu;
   std::map<std::pair<int, int>, dk::TuneEntry> tuning;  // (kw, size class) -> parameters
+  dk::DevBuf post_a, post_b;  // scratch of the table histogram / filter / query
 };
 
 namespace dk {

@@ -240,6 +240,41 @@ class KmerCounter:
             self._libh.dakc_table_free(C.byref(t))
 
     # device memory helpers
+    # after the count, on a device table (from count_device, or a served
+    # DeviceTable)
+    def table_histogram(self, t, num_buckets: int) -> tuple[np.ndarray, int]:
+        """(hist[c] for c < num_buckets, min_bucket) as construct_kmer_histogram
+        computes them (pakman_star/src/kmer_histogram.cpp:81-116)."""
+        raw = t.t if isinstance(t, DeviceTable) else t
+        hist = np.zeros(max(num_buckets, 1), dtype=np.uint64)
+        m = C.c_uint32()
+        check(self._libh.dakc_table_histogram(self._ctx, C.byref(raw), num_buckets, hist.ctypes.data,
+                                              C.byref(m)), "dakc_table_histogram")
+        return hist[:num_buckets], int(m.value)
+
+    def table_filter(self, t, min_count: int) -> None:
+        """Keep the entries with count >= min_count, in place."""
+        raw = t.t if isinstance(t, DeviceTable) else t
+        check(self._libh.dakc_table_filter(self._ctx, C.byref(raw), min_count), "dakc_table_filter")
+        if isinstance(t, DeviceTable):
+            t.n, t.total = int(raw.n), int(raw.total)
+
+    def table_query(self, t, queries: np.ndarray | None = None, *, dev_queries: int = 0, nq: int = 0,
+                    dev_out: int = 0) -> np.ndarray | None:
+        """Counts of host queries ((nq,) or (nq, key_words) u64; returns an
+        array), or of nq device queries into dev_out (returns None)."""
+        raw = t.t if isinstance(t, DeviceTable) else t
+        if dev_queries:
+            check(self._libh.dakc_table_query(self._ctx, C.byref(raw), dev_queries, nq, 1, dev_out),
+                  "dakc_table_query")
+            return None
+        q = np.ascontiguousarray(queries, dtype=np.uint64)
+        n = q.size // int(raw.key_words)
+        out = np.zeros(n, dtype=np.uint32)
+        check(self._libh.dakc_table_query(self._ctx, C.byref(raw), q.ctypes.data if n else None, n, 0,
+                                          out.ctypes.data if n else None), "dakc_table_query")
+        return out
+
     def device_alloc(self, nbytes: int) -> int:
         p = C.c_void_p()
         check(self._libh.dakc_device_alloc(self._ctx, nbytes, C.byref(p)), "dakc_device_alloc")

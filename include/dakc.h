@@ -191,6 +191,31 @@ void dakc_table_free(dakc_table *t);
  * needed when buf is NULL. */
 uint64_t dakc_table_serialize(const dakc_table *t, void *buf, uint64_t cap);
 
+/* ------------------------------------------------ after the count ---------
+ * On a device table (on_device == 1; DAKC_E_INVAL otherwise).  The reference
+ * stops at the table; these are the steps its consumers take next. */
+
+/* Count-frequency histogram: hist[c] = entries with count c for c <
+ * num_buckets (hist[0] = 0; counts >= num_buckets are not binned), and
+ * *min_bucket = the first c in [1, num_buckets) with the fewest entries (1
+ * when num_buckets == 1; 0, with no histogram, when num_buckets == 0).
+ * Replaces construct_kmer_histogram's first pass
+ * (pakman_star/src/kmer_histogram.cpp:79-116). */
+int dakc_table_histogram(dakc_ctx *ctx, const dakc_table *t, uint32_t num_buckets, uint64_t *hist,
+                         uint32_t *min_bucket);
+/* Solid-k-mer cutoff: keep the entries with count >= min_count, in order, in
+ * place; n, total (sum of the kept counts) and bin_offsets follow.  Replaces
+ * the valid-entry selection of kmer_histogram.cpp:138-140. */
+int dakc_table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_count);
+/* Point queries: counts_out[i] = count of queries[i] (key_words u64 each, lo
+ * then hi) in the table, 0 when absent; looked up in the query's owner bin
+ * when the table holds all P bins.  Queries are taken as given (a canonical
+ * table wants canonical queries).  queries and counts_out are device
+ * pointers when on_device, else host memory.  The lookup the reference
+ * describes for its sorted table (dakc/src/kcounter/kcounter.cpp:381-383). */
+int dakc_table_query(dakc_ctx *ctx, const dakc_table *t, const uint64_t *queries, uint64_t nq,
+                     int on_device, uint32_t *counts_out);
+
 /* ------------------------------------------------ synthetic reads ---------
  * Seeded, counter-based generator of reads in the fixed-width format, so the
  * same bytes can be made on the host and on the device. */

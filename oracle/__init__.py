@@ -52,6 +52,14 @@ def load():
     lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     lib.dko_free.restype = None
     lib.dko_free.argtypes = [C.c_void_p]
+    lib.dko_histogram.restype = C.c_uint32
+    lib.dko_histogram.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]
+    lib.dko_filter.restype = C.c_uint64
+    lib.dko_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_uint32, C.c_void_p,
+                               C.c_uint32, P(C.c_uint64)]
+    lib.dko_query.restype = None
+    lib.dko_query.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint32,
+                              C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = lib
     return lib
 
@@ -144,3 +152,40 @@ def murmur64a_u64(key: int, seed: int = OWNER_SEED) -> int:
 
 def murmur64a_u128(lo: int, hi: int, seed: int = OWNER_SEED) -> int:
     return int(load().dko_murmur64a_u128(lo, hi, seed))
+
+
+def histogram(counts: np.ndarray, num_buckets: int):
+    """(hist (num_buckets,) u64, min_bucket) of a table's counts."""
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    hist = np.zeros(max(num_buckets, 1), dtype=np.uint64)
+    m = load().dko_histogram(counts.ctypes.data, counts.size, num_buckets, hist.ctypes.data)
+    return hist[:num_buckets], int(m)
+
+
+def filter_table(keys: np.ndarray, counts: np.ndarray, min_count: int, bin_offsets=None):
+    """(keys, counts, total, bin_offsets) of the entries with count >= min_count."""
+    keys = np.array(keys, dtype=np.uint64, order="C", copy=True)
+    kw = keys.shape[1] if keys.ndim == 2 else 1
+    counts = np.array(counts, dtype=np.uint32, copy=True)
+    offs = None if bin_offsets is None else np.array(bin_offsets, dtype=np.uint64, copy=True)
+    total = C.c_uint64()
+    n = load().dko_filter(keys.ctypes.data, counts.ctypes.data, counts.size, kw, min_count,
+                          offs.ctypes.data if offs is not None else None,
+                          (offs.size - 1) if offs is not None else 0, C.byref(total))
+    keys = keys.reshape(-1, kw)[:n]
+    return keys, counts[:n], int(total.value), offs
+
+
+def query(keys: np.ndarray, counts: np.ndarray, queries: np.ndarray, bin_offsets=None) -> np.ndarray:
+    """Count of each query (0 when absent) in a sorted table."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    kw = keys.shape[1] if keys.ndim == 2 else 1
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    queries = np.ascontiguousarray(queries, dtype=np.uint64).reshape(-1, kw)
+    offs = None if bin_offsets is None else np.ascontiguousarray(bin_offsets, dtype=np.uint64)
+    out = np.zeros(queries.shape[0], dtype=np.uint32)
+    load().dko_query(keys.ctypes.data, counts.ctypes.data, counts.size, kw,
+                     offs.ctypes.data if offs is not None else None,
+                     (offs.size - 1) if offs is not None else 0, queries.ctypes.data, queries.shape[0],
+                     out.ctypes.data)
+    return out

@@ -445,3 +445,76 @@ int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int cano
     free(local); free(global); free(tmp); free(cnt); free(bin_start); free(args); free(th);
     return DKO_OK;
 }
+
+/* ------------------------------------------------------- after the count --- */
+
+/* Count-frequency histogram and its minimum bucket, as construct_kmer_histogram
+ * computes them (pakman_star/src/kmer_histogram.cpp:81-116): hist[c] counts the
+ * entries with count c < num_buckets (:87-92), and min_bucket is the index of
+ * std::min_element over [1, num_buckets) (:109-110): the first minimum, or
+ * num_buckets (= 1) when that range is empty.  num_buckets == 0 skips the
+ * histogram and gives min_bucket 0 (:79-81). */
+uint32_t dko_histogram(const uint32_t *counts, uint64_t d, uint32_t num_buckets, uint64_t *hist) {
+    if (num_buckets == 0) return 0;
+    for (uint32_t c = 0; c < num_buckets; c++) hist[c] = 0;
+    for (uint64_t i = 0; i < d; i++)
+        if (counts[i] < num_buckets) hist[counts[i]]++;
+    uint32_t m = 1;
+    for (uint32_t c = 2; c < num_buckets; c++)
+        if (hist[c] < hist[m]) m = c;
+    return m;
+}
+
+/* Keep the entries with count >= min_count, in table order, in place
+ * (kmer_histogram.cpp:138-140 selects k_count >= min_bucket).  Returns the kept
+ * count; *total_out = sum of the kept counts; bin_offsets (num_bins + 1
+ * entries, or NULL) become the kept entries' bin boundaries. */
+uint64_t dko_filter(uint64_t *keys, uint32_t *counts, uint64_t d, int key_words, uint32_t min_count,
+                    uint64_t *bin_offsets, uint32_t num_bins, uint64_t *total_out) {
+    uint64_t o = 0, total = 0, b = 0;
+    for (uint64_t i = 0; i <= d; i++) {
+        /* boundaries at position i move to the kept entries before i */
+        while (bin_offsets && b <= num_bins && bin_offsets[b] == i) bin_offsets[b++] = o;
+        if (i == d) break;
+        if (counts[i] < min_count) continue;
+        for (int x = 0; x < key_words; x++) keys[o * key_words + x] = keys[i * key_words + x];
+        counts[o] = counts[i];
+        total += counts[i];
+        o++;
+    }
+    if (total_out) *total_out = total;
+    return o;
+}
+
+static int key_cmp(const uint64_t *a, const uint64_t *b, int kw) {
+    for (int x = kw - 1; x >= 0; x--) /* the high word decides first */
+        if (a[x] != b[x]) return a[x] < b[x] ? -1 : 1;
+    return 0;
+}
+
+/* Count of each query in a sorted table (0 when absent), by binary search in
+ * the query's owner bin (MurmurHash64A % num_bins, kcounter.cpp:50-54) when
+ * bin_offsets is given, else in the whole table: the lookup kcounter.cpp:381-383
+ * describes for the sorted lightdbg. */
+void dko_query(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words,
+               const uint64_t *bin_offsets, uint32_t num_bins, const uint64_t *queries, uint64_t nq,
+               uint32_t *out) {
+    for (uint64_t i = 0; i < nq; i++) {
+        const uint64_t *q = queries + i * key_words;
+        uint64_t lo = 0, hi = d;
+        if (bin_offsets && num_bins) {
+            uint64_t h = key_words == 1 ? dko_murmur64a_u64(q[0], DKO_OWNER_SEED)
+                                        : dko_murmur64a_u128(q[0], q[1], DKO_OWNER_SEED);
+            uint32_t o = (uint32_t)(h % num_bins);
+            lo = bin_offsets[o];
+            hi = bin_offsets[o + 1];
+        }
+        out[i] = 0;
+        while (lo < hi) {
+            uint64_t mid = lo + (hi - lo) / 2;
+            int c = key_cmp(keys + mid * key_words, q, key_words);
+            if (c == 0) { out[i] = counts[mid]; break; }
+            if (c < 0) lo = mid + 1; else hi = mid;
+        }
+    }
+}

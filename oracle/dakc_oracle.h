@@ -84,6 +84,14 @@ uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64
 
 void dko_free(void *p);
 
+/* After the count (pakman_star/src/kmer_histogram.cpp:79-140; kcounter.cpp:381-383). */
+uint32_t dko_histogram(const uint32_t *counts, uint64_t d, uint32_t num_buckets, uint64_t *hist);
+uint64_t dko_filter(uint64_t *keys, uint32_t *counts, uint64_t d, int key_words, uint32_t min_count,
+                    uint64_t *bin_offsets, uint32_t num_bins, uint64_t *total_out);
+void dko_query(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words,
+               const uint64_t *bin_offsets, uint32_t num_bins, const uint64_t *queries, uint64_t nq,
+               uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

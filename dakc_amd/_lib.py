@@ -20,7 +20,7 @@ STATUS_NAMES = {
     5: "DAKC_E_DEVICE", 6: "DAKC_E_NODEVICE", 7: "DAKC_E_REJECTED", 8: "DAKC_E_NOTFOUND",
     9: "DAKC_E_PENDING", 10: "DAKC_E_CANCELLED",
 }
-FMT_FIXED = 0
+FMT_FIXED, FMT_LINES, FMT_FASTQ = 0, 1, 2
 ALGO_AUTO, ALGO_LSD, ALGO_MSD = 0, 1, 2
 
 # every symbol include/dakc.h declares
@@ -32,6 +32,7 @@ EXPORTS = [
     "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
+    "dakc_convert_reads",
 ]
 
 
@@ -142,6 +143,8 @@ def load() -> C.CDLL:
         "dakc_table_histogram": (C.c_int, [V, P(Table), C.c_uint32, V, P(C.c_uint32)]),
         "dakc_table_filter": (C.c_int, [V, P(Table), C.c_uint32]),
         "dakc_table_query": (C.c_int, [V, P(Table), V, C.c_uint64, C.c_int, V]),
+        "dakc_convert_reads": (C.c_int, [V, V, C.c_uint64, C.c_int, C.c_uint32, V, C.c_uint64,
+                                         P(C.c_uint64), P(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

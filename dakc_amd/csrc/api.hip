@@ -757,8 +757,7 @@ extern "C" int dakc_ctx_destroy(dakc_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->ws.~JobWs();
   new (&c->ws) JobWs();
-  c->post_a.release();
-  c->post_b.release();
+  for (DevBuf *b : {&c->post_a, &c->post_b, &c->conv_in, &c->conv_nl, &c->conv_out}) b->release();
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return DAKC_OK;
@@ -800,10 +799,42 @@ static void fill_stats(dakc_stats &st, JobWs &ws, const Plan &pl, uint64_t n, ui
   st.total_ms = ms;
 }
 
+// FASTQ or line input: convert it on the device into ctx->conv_out and
+// describe that as a fixed-width job (host bytes are staged in ctx->conv_in).
+static int convert_job(dakc_ctx *c, const dakc_job *job, dakc_job *fixed) {
+  if (job->read_len < 0 || (job->nbytes && !job->reads)) return DAKC_E_INVAL;
+  const uint8_t *in = (const uint8_t *)job->reads;
+  int s;
+  if (!job->reads_on_device && job->nbytes) {
+    if ((s = c->conv_in.ensure(job->nbytes, c->stream, false))) return s;
+    if (hipMemcpyAsync(c->conv_in.p, job->reads, job->nbytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return DAKC_E_DEVICE;
+    in = (const uint8_t *)c->conv_in.p;
+  }
+  uint64_t nb = 0;
+  uint32_t L = 0;
+  if ((s = convert_reads(c, in, job->nbytes, job->format, (uint32_t)job->read_len, nullptr, 0, &c->conv_out,
+                         &nb, &L)))
+    return s;
+  *fixed = *job;
+  fixed->format = DAKC_FMT_FIXED;
+  fixed->reads = (const char *)c->conv_out.p;
+  fixed->nbytes = nb;
+  fixed->read_len = (int32_t)L;
+  fixed->reads_on_device = 1;
+  return DAKC_OK;
+}
+
 static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool to_host) {
   if (!c || !job || !out) return DAKC_E_INVAL;
   std::memset(out, 0, sizeof(*out));
   hipSetDevice(c->device);
+  dakc_job fixed;
+  if (job->format == DAKC_FMT_FASTQ || job->format == DAKC_FMT_LINES) {
+    int s = convert_job(c, job, &fixed);
+    if (s) return s;
+    job = &fixed;
+  }
   Plan pl;
   int s = make_plan(job, &pl);
   if (s) return s;

@@ -181,10 +181,13 @@ struct dakc_ctx {
   std::mutex tune_mu;
   std::map<std::pair<int, int>, dk::TuneEntry> tuning;  // (kw, size class) -> parameters
   dk::DevBuf post_a, post_b;  // scratch of the table histogram / filter / query
+  dk::DevBuf conv_in, conv_nl, conv_out;  // FASTQ / line input: staged bytes, newlines, records
 };
 
 namespace dk {
 int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *outs);
+int convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int format, uint32_t read_len,
+                  uint8_t *out, uint64_t cap, DevBuf *grow, uint64_t *out_nbytes, uint32_t *out_read_len);
 void serve_shutdown(dakc_ctx *c);
 
 // Owner of a device-resident result table (dakc_table.priv): the two ranges

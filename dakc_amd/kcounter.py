@@ -113,9 +113,10 @@ class KmerCounter:
 
     def job(self, reads_ptr: int, nbytes: int, *, on_device: bool = False, **over) -> Job:
         p = dict(k=self.k, read_len=self.read_len, canonical=self.canonical,
-                 num_pes=self.num_pes, owner_bin=self.owner_bin, table_on_device=False)
+                 num_pes=self.num_pes, owner_bin=self.owner_bin, table_on_device=False,
+                 format=_lib.FMT_FIXED)
         p.update(over)
-        return Job(reads=reads_ptr, nbytes=nbytes, format=_lib.FMT_FIXED, read_len=p["read_len"],
+        return Job(reads=reads_ptr, nbytes=nbytes, format=p["format"], read_len=p["read_len"],
                    k=p["k"], canonical=1 if p["canonical"] else 0, num_pes=p["num_pes"],
                    owner_bin=p["owner_bin"], reads_on_device=1 if on_device else 0,
                    table_on_device=1 if p["table_on_device"] else 0)
@@ -240,6 +241,22 @@ class KmerCounter:
             self._libh.dakc_table_free(C.byref(t))
 
     # device memory helpers
+    # before the count: FASTQ / line input to fixed-width records on the device
+    def convert_reads(self, dev_ptr: int, nbytes: int, fmt: int, read_len: int = 0) -> tuple[int, int, int]:
+        """(device pointer, bytes, width) of the fixed-width records made from
+        FASTQ (fmt=FMT_FASTQ) or line (FMT_LINES) input in device memory, as
+        syngen/fq2txtmaker.sh makes them; free the pointer with device_free."""
+        n, L = C.c_uint64(), C.c_uint32()
+        check(self._libh.dakc_convert_reads(self._ctx, dev_ptr or None, nbytes, fmt, read_len, None, 0,
+                                            C.byref(n), C.byref(L)), "dakc_convert_reads")
+        out = self.device_alloc(max(int(n.value), 4))
+        st = self._libh.dakc_convert_reads(self._ctx, dev_ptr or None, nbytes, fmt, read_len, out,
+                                           max(int(n.value), 4), C.byref(n), C.byref(L))
+        if st:
+            self.device_free(out)
+            check(st, "dakc_convert_reads")
+        return out, int(n.value), int(L.value)
+
     # after the count, on a device table (from count_device, or a served
     # DeviceTable)
     def table_histogram(self, t, num_buckets: int) -> tuple[np.ndarray, int]:

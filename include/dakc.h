@@ -52,12 +52,17 @@ typedef enum dakc_status {
 
 /* read buffer formats */
 #define DAKC_FMT_FIXED 0 /* read_len bytes + '\n' per record (the reference's .txt) */
+#define DAKC_FMT_LINES 1 /* one read per '\n'-ended line, any lengths */
+#define DAKC_FMT_FASTQ 2 /* 4-line FASTQ records; the second line of each is the read */
 
 typedef struct dakc_job {
   const char *reads;   /* host pointer, or device pointer when reads_on_device */
   uint64_t nbytes;
-  int32_t format;      /* DAKC_FMT_FIXED */
-  int32_t read_len;    /* READLEN: record width in bases (1..4096) */
+  int32_t format;      /* DAKC_FMT_*; LINES and FASTQ are converted on the device
+                        * (dakc_convert_reads) by dakc_count / dakc_count_device
+                        * only; batched and served jobs take FIXED */
+  int32_t read_len;    /* READLEN: record width in bases (1..4096); for LINES /
+                        * FASTQ 0 = the longest read */
   int32_t k;           /* KMERLEN: 1..64 (k <= 32: one u64 per key, else two) */
   int32_t canonical;   /* 0: forward strand (the reference), 1: min(x, revcomp x) */
   uint32_t num_pes;    /* P; 1 = single PE */
@@ -190,6 +195,17 @@ void dakc_table_free(dakc_table *t);
  * u64 lo, u64 hi (k <= 64), then u32 count.  Returns bytes written or
  * needed when buf is NULL. */
 uint64_t dakc_table_serialize(const dakc_table *t, void *buf, uint64_t cap);
+
+/* ------------------------------------------------ before the count --------
+ * FASTQ (DAKC_FMT_FASTQ) or newline-delimited reads (DAKC_FMT_LINES) in device
+ * memory (4-byte aligned) become fixed-width records on the device, as
+ * syngen/fq2txtmaker.sh:12-18 makes them with awk: the read line of each
+ * record (every line for LINES), padded with 'M' to read_len (0: the longest
+ * read; a longer read is DAKC_E_FORMAT), every space made 'M', each followed
+ * by '\n'.  out (device, cap bytes) receives *out_nbytes bytes of records
+ * of width *out_read_len; with out == NULL only the sizes are returned. */
+int dakc_convert_reads(dakc_ctx *ctx, const void *in, uint64_t nbytes, int format, uint32_t read_len,
+                       void *out, uint64_t cap, uint64_t *out_nbytes, uint32_t *out_read_len);
 
 /* ------------------------------------------------ after the count ---------
  * On a device table (on_device == 1; DAKC_E_INVAL otherwise).  The reference

@@ -52,6 +52,9 @@ def load():
     lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     lib.dko_free.restype = None
     lib.dko_free.argtypes = [C.c_void_p]
+    lib.dko_convert.restype = C.c_int
+    lib.dko_convert.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
+                                P(C.c_uint64), P(C.c_uint32)]
     lib.dko_histogram.restype = C.c_uint32
     lib.dko_histogram.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]
     lib.dko_filter.restype = C.c_uint64
@@ -189,3 +192,20 @@ def query(keys: np.ndarray, counts: np.ndarray, queries: np.ndarray, bin_offsets
                      (offs.size - 1) if offs is not None else 0, queries.ctypes.data, queries.shape[0],
                      out.ctypes.data)
     return out
+
+
+def convert(data, fastq: bool, read_len: int = 0):
+    """(fixed-width record bytes as uint8 array, width) of FASTQ or line input."""
+    lib = load()
+    b = _buf(data)
+    n, L = C.c_uint64(), C.c_uint32()
+    p = b.ctypes.data if b.size else None
+    st = lib.dko_convert(p, b.size, 1 if fastq else 0, read_len, None, 0, C.byref(n), C.byref(L))
+    if st:
+        raise OracleError(st)
+    out = np.zeros(max(int(n.value), 1), dtype=np.uint8)
+    st = lib.dko_convert(p, b.size, 1 if fastq else 0, read_len, out.ctypes.data, out.size, C.byref(n),
+                         C.byref(L))
+    if st:
+        raise OracleError(st)
+    return out[:int(n.value)], int(L.value)

@@ -518,3 +518,51 @@ void dko_query(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key
         }
     }
 }
+
+/* ------------------------------------------------------ before the count --- */
+
+/* FASTQ (fastq != 0) or newline-delimited reads to the fixed-width format, as
+ * syngen/fq2txtmaker.sh does with awk: the lines with NR % 4 == 2 (:12; every
+ * line for plain reads), where a line ends at '\n' and a last line without
+ * its '\n' still counts; the longest such line sets the width (:15); each line
+ * is left-justified in that width with spaces and every space becomes 'M'
+ * (:18, printf "%-Ns" | sed 's/ /M/g'), then '\n'.  read_len > 0 sets the
+ * width instead (a longer line is DKO_E_FORMAT); the width is at least 1.
+ * out == NULL: sizes only.  *out_n = bytes written, *width = record width. */
+int dko_convert(const char *in, size_t n, int fastq, int read_len, char *out, size_t cap,
+                uint64_t *out_n, uint32_t *width) {
+    if (read_len < 0 || read_len > 4096) return DKO_E_INVAL;
+    uint64_t longest = 0, nrec = 0, line = 0;
+    for (size_t s = 0; s < n; line++) {
+        size_t e = s;
+        while (e < n && in[e] != '\n') e++;
+        if (!fastq || line % 4 == 1) {
+            if (e - s > longest) longest = e - s;
+            nrec++;
+        }
+        s = e + 1;
+    }
+    if (read_len && longest > (uint64_t)read_len) return DKO_E_FORMAT;
+    if (!read_len && longest > 4096) return DKO_E_TOOBIG;
+    uint32_t L = read_len ? (uint32_t)read_len : (longest ? (uint32_t)longest : 1u);
+    uint64_t need = nrec * ((uint64_t)L + 1);
+    if (out_n) *out_n = need;
+    if (width) *width = L;
+    if (!out) return DKO_OK;
+    if (cap < need) return DKO_E_INVAL;
+    uint64_t o = 0;
+    line = 0;
+    for (size_t s = 0; s < n; line++) {
+        size_t e = s;
+        while (e < n && in[e] != '\n') e++;
+        if (!fastq || line % 4 == 1) {
+            for (uint32_t p = 0; p < L; p++) {
+                char c = s + p < e ? in[s + p] : ' ';
+                out[o++] = c == ' ' ? 'M' : c;
+            }
+            out[o++] = '\n';
+        }
+        s = e + 1;
+    }
+    return DKO_OK;
+}

@@ -84,6 +84,10 @@ uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64
 
 void dko_free(void *p);
 
+/* Before the count: FASTQ / line input to fixed-width records (syngen/fq2txtmaker.sh:12-18). */
+int dko_convert(const char *in, size_t n, int fastq, int read_len, char *out, size_t cap,
+                uint64_t *out_n, uint32_t *width);
+
 /* After the count (pakman_star/src/kmer_histogram.cpp:79-140; kcounter.cpp:381-383). */
 uint32_t dko_histogram(const uint32_t *counts, uint64_t d, uint32_t num_buckets, uint64_t *hist);
 uint64_t dko_filter(uint64_t *keys, uint32_t *counts, uint64_t d, int key_words, uint32_t min_count,

@@ -74,6 +74,37 @@ def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
                       f"{t:.3f} s each"}
 
 
+def pre_count(kc, dreads: int, w: dict, nreads: int = 4_000_000) -> dict:
+    """FASTQ of the first `nreads` reads of the workload (header "@r", the read,
+    "+", a quality line), converted on the device to fixed-width records (one
+    untimed call, then one timed); the records must equal the generator's."""
+    from dakc_amd import _lib
+    L = w["read_len"]
+    nreads = min(nreads, w["num_reads"])
+    fixed = kc.d2h(dreads, nreads * (L + 1), np.uint8).reshape(nreads, L + 1)
+    rec = np.empty((nreads, 3 + (L + 1) + 2 + (L + 1)), dtype=np.uint8)
+    rec[:, 0:3] = np.frombuffer(b"@r\n", np.uint8)
+    rec[:, 3:4 + L] = fixed
+    rec[:, 4 + L:6 + L] = np.frombuffer(b"+\n", np.uint8)
+    rec[:, 6 + L:6 + 2 * L] = ord("F")
+    rec[:, 6 + 2 * L] = ord("\n")
+    fq = rec.reshape(-1)
+    dfq = kc.device_alloc(fq.size)
+    kc.h2d(dfq, fq)
+    out, n, width = kc.convert_reads(dfq, fq.size, _lib.FMT_FASTQ)
+    kc.device_free(out)
+    t0 = time.perf_counter()
+    out, n, width = kc.convert_reads(dfq, fq.size, _lib.FMT_FASTQ)
+    s = time.perf_counter() - t0
+    same = width == L and n == fixed.size and np.array_equal(kc.d2h(out, n, np.uint8), fixed.reshape(-1))
+    kc.device_free(out)
+    kc.device_free(dfq)
+    return {"format": "fastq", "reads": nreads, "in_bytes": int(fq.size), "out_bytes": int(n), "ms": s * 1e3,
+            "GBps_in_plus_out": (fq.size + n) / s / 1e9, "records_equal_generator": bool(same),
+            "timing": "wall clock around the synchronous call (newline count, offsets, longest read, "
+                      "records), one run, not in the timed counts"}
+
+
 def post_count(kc, t, kb: int) -> dict:
     """After the count, on its device table `t` (once each, after one untimed
     call): the 64-bucket count histogram, point queries (5M keys taken from
@@ -356,7 +387,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-post", action="store_true",
-                    help="skip the after-the-count histogram / query / filter measurement")
+                    help="skip the FASTQ conversion and the after-the-count histogram / query / "
+                         "filter measurements")
     ap.add_argument("--algo", default="auto", choices=["auto", "lsd"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also copy the read buffer device-to-device once (a plain copy kernel "
@@ -439,6 +471,7 @@ def main() -> int:
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     traffic = measured_traffic(a.workload, dom)
     post = post_count(kc, t, kb) if not a.no_post else None
+    pre = pre_count(kc, dreads, w) if not a.no_post else None
     stages = {
         "path": "msd" if path == 2 else "lsd", "msd_reruns": reruns, "distinct": d,
         "kernels": {name: {"avg_ms": t_ms, "alg_bytes": nb_,
@@ -470,6 +503,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
+        "pre_count": pre,
         "post_count": post,
     }
     kc.device_free(dreads)

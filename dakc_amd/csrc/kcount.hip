@@ -1198,7 +1198,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
                          uint32_t *s_total, uint32_t *s_fail, uint64_t *s_excl, uint32_t sb,
                          uint32_t nsb) {
   constexpr int SPT = HT / NT;  // table slots per thread
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = opaque_tid();
   uint64_t *tk = lds;                                     // HT keys, then the ordered keys
   uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);  // HT counts, then the ordered counts
   uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
@@ -1310,8 +1310,10 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
+// Four waves per SIMD (128 VGPRs at most): one 1024-, two 512- or four
+// 256-thread workgroups per CU.
 template <typename KeyT, int NT, int ITEMS, int NB3, int HT, bool ONE>
-__global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
+__global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   constexpr int CAP = NT * ITEMS;
   constexpr int TPT = FIN_MAX_GROUPS / NT;
   __shared__ KeyT B[CAP];
@@ -1321,141 +1323,196 @@ __global__ __launch_bounds__(NT) void k_fin(BatchArgs a, JobDesc one) {
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
   __shared__ uint32_t s_mixed[NB3 / 32];
-  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job, s_fail;
+  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job, s_fail, s_next;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
-  __syncthreads();
-  const uint32_t item = a.item0 + __builtin_amdgcn_readfirstlane(s_sb);
-  const uint32_t job = ONE ? 0u : find_job_wg(a.pre3, a.njobs, item, &s_job);
-  const JobDesc J = ONE ? one : job_desc(a, job);
-  if (tid == 0) {
+
+  // the overflow counters of the distinct-key path (fail, ok) of a job
+  auto fail_word = [](const JobDesc &J) {
+    return __hip_atomic_load(reinterpret_cast<const uint64_t *>(reinterpret_cast<const uint8_t *>(J.nd) + 24),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  // group rows of sub-bucket sb (loads left in flight)
+  auto rows_of = [&](const MsdArgs &m, uint32_t sb, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT]) {
+    const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
+    const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
+    load_rows<TPT>(row0, row0 + m.ngroups, m.ngroups, r0, r1);
+  };
+
+  // One sub-bucket sb of job J, its group rows in r0/r1 and the job's
+  // overflow counters in fo (tid 0).  `between` runs right after the gather,
+  // when the rows are free again; every thread returns together.
+  auto run = [&](const JobDesc &J, uint32_t sb, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT], uint64_t fo,
+                 auto &&between) {
+    const uint32_t tid = opaque_tid();  // per item: no address is kept across items
     // distinct-key path: skip it once most earlier sub-buckets of the job
-    // overflowed the table (read here, before the key loads are in flight)
-    const uint64_t fo = __hip_atomic_load(reinterpret_cast<const uint64_t *>(
-                                              reinterpret_cast<const uint8_t *>(J.nd) + 24),
-                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_fail = (uint32_t)fo > (uint32_t)(fo >> 32) + 32 ? 1u : 0u;
-  }
-  const MsdArgs m = J.m;
-  const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
-  KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
-  uint32_t *out_counts = J.tcounts;
-  uint64_t *n_dev = J.nd;
-  uint32_t *err = J.err;
-  const uint32_t nd3 = 1u << m.d3.nbits;
-  for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
-  for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
-  __syncthreads();
-  const uint32_t sb = item - (ONE ? 0u : pre_at(a.pre3, job)), nsb = m.nb1 * m.nd2;
-  const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
-  const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
+    // overflowed the table
+    if (tid == 0) s_fail = (uint32_t)fo > (uint32_t)(fo >> 32) + 32 ? 1u : 0u;
+    const MsdArgs m = J.m;
+    const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
+    KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
+    uint32_t *out_counts = J.tcounts;
+    uint64_t *n_dev = J.nd;
+    uint32_t *err = J.err;
+    const uint32_t nd3 = 1u << m.d3.nbits;
+    for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
+    for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
+    __syncthreads();
+    const uint32_t nsb = m.nb1 * m.nd2;
+    const uint32_t b = sb / m.nd2;
 
-  KeyT keys[ITEMS];
-  uint32_t mine;
-  uint32_t r0[TPT], r1[TPT];
-  load_rows<TPT>(row0, row0 + m.ngroups, m.ngroups, r0, r1);
-  const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
-      in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
-      keys, mine);
-  if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
-  const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
+    KeyT keys[ITEMS];
+    uint32_t mine;
+    const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
+        in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
+        keys, mine);
+    between();
+    if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
+    const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
 
-  if constexpr (sizeof(KeyT) == 8) {
-    static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
-    if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
-                                                       s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
-                                                       nsb))
-      return;
-  }
-
-  // split by the next b3 bits into B (unstable, LDS atomics)
-  uint32_t dig[ITEMS], rank[ITEMS];
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    if ((uint32_t)u < mine) {
-      dig[u] = key_bits(keys[u], m.d3.shift, m.d3.nbits);
-      rank[u] = atomicAdd(&s_h[dig[u]], 1u);
+    if constexpr (sizeof(KeyT) == 8) {
+      static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
+      if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
+                                                         s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
+                                                         nsb))
+        return;
     }
-  }
-  __syncthreads();
-  scan_lds<NT>(s_h, nd3, s_wave, &s_total);
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u)
-    if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
-  __syncthreads();
 
-  // Order each bin by counting: a key's place is the number of keys of its
-  // bin that are smaller, plus the equal ones stored before it; reads are
-  // independent and copies of one k-mer keep their order.  Bins whose keys
-  // are all equal are already in order; a mixed bin longer than RANK_MAX flags
-  // the job, which is then recounted on the LSD path.
-  constexpr uint32_t RANK_MAX = 128;
-#pragma unroll
-  for (int q = 0; q < ITEMS; ++q) {
-    const uint32_t i = tid + q * NT;
-    if (i < mcnt) {
-      const KeyT x = B[i];
-      keys[q] = x;
-      dig[q] = key_bits(x, m.d3.shift, m.d3.nbits);
-      if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
-    }
-  }
-  __syncthreads();
-  uint32_t newpos[ITEMS];
-  bool bad = false;
-#pragma unroll
-  for (int q = 0; q < ITEMS; ++q) {
-    const uint32_t i = tid + q * NT;
-    newpos[q] = i;
-    if (i < mcnt && ((s_mixed[dig[q] >> 5] >> (dig[q] & 31)) & 1)) {
-      const KeyT x = keys[q];
-      const uint32_t lo = s_h[dig[q]], hi = s_h[dig[q] + 1];
-      if (hi - lo <= RANK_MAX) {
-        // eight independent LDS reads per step; past the bin end y = x, which
-        // counts neither as smaller nor as an earlier copy (t > i there)
-        uint32_t r = 0;
-        for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
-          KeyT y[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) y[j] = t0 + j < hi ? B[t0 + j] : x;
-#pragma unroll
-          for (int j = 0; j < 8; ++j) r += key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x));
-        }
-        newpos[q] = lo + r;
-      } else {
-        bad = true;
+    // split by the next b3 bits into B (unstable, LDS atomics)
+    uint32_t dig[ITEMS], rank[ITEMS];
+  #pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      if ((uint32_t)u < mine) {
+        dig[u] = key_bits(keys[u], m.d3.shift, m.d3.nbits);
+        rank[u] = atomicAdd(&s_h[dig[u]], 1u);
       }
     }
-  }
-  if (bad) atomicOr(err, ERR_OVERFLOW);
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < ITEMS; ++q)
-    if (tid + q * NT < mcnt) B[newpos[q]] = keys[q];
-  __syncthreads();
+    __syncthreads();
+    scan_lds<NT>(s_h, nd3, s_wave, &s_total);
+  #pragma unroll
+    for (int u = 0; u < ITEMS; ++u)
+      if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
+    __syncthreads();
 
-  // run-length reduce inside LDS
-  uint32_t heads = 0, hmask = 0;
-#pragma unroll
-  for (int q = 0; q < ITEMS; ++q) {
-    const uint32_t i = tid * ITEMS + q;
-    const bool h = i < mcnt && (i == 0 || !key_eq(B[i], B[i - 1]));
-    hmask |= (h ? 1u : 0u) << q;
-    heads += h;
-  }
-  uint32_t o = scan_block<NT>(heads, s_wave, &s_total);
-  const uint32_t H = s_total;
-  uint16_t *s_hp = s_u.hp;
-#pragma unroll
-  for (int q = 0; q < ITEMS; ++q)
-    if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
-  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, &s_excl);
-  for (uint32_t h = tid; h < H; h += NT) {
-    const uint32_t i = s_hp[h];
-    const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;
-    out_keys[excl + h] = B[i];
-    out_counts[excl + h] = end - i;
+    // Order each bin by counting: a key's place is the number of keys of its
+    // bin that are smaller, plus the equal ones stored before it; reads are
+    // independent and copies of one k-mer keep their order.  Bins whose keys
+    // are all equal are already in order; a mixed bin longer than RANK_MAX flags
+    // the job, which is then recounted on the LSD path.
+    constexpr uint32_t RANK_MAX = 128;
+  #pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const uint32_t i = tid + q * NT;
+      if (i < mcnt) {
+        const KeyT x = B[i];
+        keys[q] = x;
+        dig[q] = key_bits(x, m.d3.shift, m.d3.nbits);
+        if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
+      }
+    }
+    __syncthreads();
+    uint32_t newpos[ITEMS];
+    bool bad = false;
+  #pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const uint32_t i = tid + q * NT;
+      newpos[q] = i;
+      if (i < mcnt && ((s_mixed[dig[q] >> 5] >> (dig[q] & 31)) & 1)) {
+        const KeyT x = keys[q];
+        const uint32_t lo = s_h[dig[q]], hi = s_h[dig[q] + 1];
+        if (hi - lo <= RANK_MAX) {
+          // eight independent LDS reads per step; past the bin end y = x, which
+          // counts neither as smaller nor as an earlier copy (t > i there)
+          uint32_t r = 0;
+          for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+            KeyT y[8];
+  #pragma unroll
+            for (int j = 0; j < 8; ++j) y[j] = t0 + j < hi ? B[t0 + j] : x;
+  #pragma unroll
+            for (int j = 0; j < 8; ++j) r += key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x));
+          }
+          newpos[q] = lo + r;
+        } else {
+          bad = true;
+        }
+      }
+    }
+    if (bad) atomicOr(err, ERR_OVERFLOW);
+    __syncthreads();
+  #pragma unroll
+    for (int q = 0; q < ITEMS; ++q)
+      if (tid + q * NT < mcnt) B[newpos[q]] = keys[q];
+    __syncthreads();
+
+    // run-length reduce inside LDS
+    uint32_t heads = 0, hmask = 0;
+  #pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const uint32_t i = tid * ITEMS + q;
+      const bool h = i < mcnt && (i == 0 || !key_eq(B[i], B[i - 1]));
+      hmask |= (h ? 1u : 0u) << q;
+      heads += h;
+    }
+    uint32_t o = scan_block<NT>(heads, s_wave, &s_total);
+    const uint32_t H = s_total;
+    uint16_t *s_hp = s_u.hp;
+  #pragma unroll
+    for (int q = 0; q < ITEMS; ++q)
+      if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
+    const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, &s_excl);
+    for (uint32_t h = tid; h < H; h += NT) {
+      const uint32_t i = s_hp[h];
+      const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;
+      out_keys[excl + h] = B[i];
+      out_counts[excl + h] = end - i;
+    }
+  };
+
+  if constexpr (ONE) {
+    // One job: persistent over the tickets.  The next ticket is taken before
+    // this sub-bucket's gather and read after it, and the next sub-bucket's
+    // group rows and overflow counters load while this one is sorted and
+    // written.  Tickets keep the look-back's predecessors running.
+    const uint32_t n = a.nfin;
+    if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    uint32_t t = __builtin_amdgcn_readfirstlane(s_sb);
+    uint32_t r0[TPT], r1[TPT];
+    uint64_t fo = 0;
+    if (t < n) {
+      rows_of(one.m, t, r0, r1);
+      if (tid == 0) fo = fail_word(one);
+    }
+    while (t < n) {
+      uint32_t nt = 0;
+      if (tid == 0) nt = atomicAdd(a.ticket, 1u);  // its value is needed only after the gather
+      uint32_t tn = n;
+      uint64_t fon = 0;
+      run(one, t, r0, r1, fo, [&] {
+        if (tid == 0) s_next = nt;
+        __syncthreads();
+        tn = __builtin_amdgcn_readfirstlane(s_next);
+        if (tn < n) {
+          rows_of(one.m, tn, r0, r1);
+          if (tid == 0) fon = fail_word(one);
+        }
+      });
+      __syncthreads();  // this sub-bucket's LDS is free, s_next read by all
+      t = tn;
+      fo = fon;
+    }
+  } else {
+    // several jobs: one sub-bucket per workgroup, its job looked up
+    if (tid == 0) s_sb = atomicAdd(a.ticket, 1u);  // items in ticket order, job by job
+    __syncthreads();
+    const uint32_t item = a.item0 + __builtin_amdgcn_readfirstlane(s_sb);
+    const uint32_t job = find_job_wg(a.pre3, a.njobs, item, &s_job);
+    const JobDesc J = job_desc(a, job);
+    uint32_t r0[TPT], r1[TPT];
+    const uint32_t sb = item - pre_at(a.pre3, job);
+    const uint64_t fo = tid == 0 ? fail_word(J) : 0;
+    rows_of(J.m, sb, r0, r1);
+    run(J, sb, r0, r1, fo, [] {});
   }
 }
 
@@ -1562,7 +1619,14 @@ static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint
   if (n1) k_l1<KeyT, MSD_NT, ITEMS, ONE><<<std::min(n1, device_cus(st)), MSD_NT, 0, st>>>(a, one);
   if (n2) k_l2<KeyT, L2_NT, ITEMS, ONE><<<std::min(n2, device_cus(st)), L2_NT, 0, st>>>(a, one);
   constexpr int FI = sizeof(KeyT) == 8 ? FIN_ITEMS_1 : FIN_ITEMS_2;
-  if (n3) k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<n3, NT3, 0, st>>>(a, one);
+  if (n3) {
+    // one job: persistent workgroups (as many as fit beside each other, by
+    // thread count) over the tickets; several jobs: one workgroup per item
+    BatchArgs af = a;
+    af.nfin = n3;
+    const uint32_t grid = ONE ? std::min(n3, device_cus(st) * (uint32_t)(1024 / NT3)) : n3;
+    k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<grid, NT3, 0, st>>>(af, one);
+  }
 }
 
 template <typename KeyT, int ITEMS, bool ONE>

@@ -326,12 +326,17 @@ int make_plan(const dakc_job *job, Plan *pl) {
   return DAKC_OK;
 }
 
-// MSD geometry.  A final sub-bucket should hold at most tune_fill_pct (60 by
-// default) percent of what one final workgroup sorts in LDS, which fixes b1 + b2; b1 <= 10 (1024 first-level
-// buckets, fewer key bits when owner bins share them) and b2 <= 10.  L1 tiles
-// take as many records as fill 15360 (8192) slots; an L2 group gathers G
+// MSD geometry.  A final item should hold at most tune_fill_pct (90 by
+// default) percent of what one final workgroup sorts in LDS on average.  The
+// coarsest split that meets it fixes b1 (<= 10: 1024 first-level buckets,
+// fewer key bits when owner bins share them); the second level then splits up
+// to two bits finer (b2 <= 10) and a final item takes `pack` consecutive
+// second-level digits, as many as the target holds, so items come close to
+// the target instead of to the nearest power of two below it.  L1 tiles take
+// as many records as fill 15360 (8192) slots; an L2 group gathers G
 // consecutive L1 tiles of a bucket, G sized so a group fills 75% of an L2
-// workgroup on average.  b3 (<= 10) splits a sub-bucket into LDS bins.
+// workgroup on average.  b3 splits each digit of an item into LDS bins (pack
+// << b3 bins in all).
 int plan_choose_path(Plan *pl, int algo) {
   Plan &p = *pl;
   p.msd = 0;
@@ -340,7 +345,7 @@ int plan_choose_path(Plan *pl, int algo) {
   const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : p.tune_fin_nt == 1024 ? 1024u : 512u;
   const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap1 = (uint64_t)msd_tile(p.kw),
                  cap2 = (uint64_t)l2_cap(p.kw);
-  const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 60) / 100;
+  const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100;
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
   const uint32_t owners = p.owner_digit ? p.num_pes : 1;
@@ -360,18 +365,29 @@ int plan_choose_path(Plan *pl, int algo) {
   rt = std::min<uint32_t>(rt, L1_CODEW / wpr);
   rt = std::max<uint32_t>(rt, 1);
   if ((uint64_t)rt * p.S > cap1) return 0;
+  const uint32_t b2f = std::min(std::min(10u, bits - b1), b2 + 2);
+  const double per_digit = (double)p.nmax / (double)(1ull << (b1 + b2f));
+  uint32_t pack = (uint32_t)std::max(1.0, std::floor((double)target / std::max(per_digit, 1e-9)));
+  pack = std::min<uint32_t>(std::min<uint32_t>(pack, 16), 1u << b2f);
+  const uint32_t nb3 = fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup
+  uint32_t b3 = 0;
+  while (b3 < 10 && b3 < bits - b1 - b2f && (pack << (b3 + 1)) <= nb3) ++b3;
   p.b1 = b1;
-  p.b2 = b2;
-  p.b3 = std::min<uint32_t>(fin_nt >= 512 ? 10 : 9, bits - b1 - b2);
+  p.b2 = b2f;
+  p.b3 = b3;
+  p.pack = pack;
   p.fin_nt = fin_nt;
   p.nb1 = owners << b1;
-  p.nd2 = 1u << b2;
-  p.nsb = p.nb1 * p.nd2;
+  p.nd2 = 1u << b2f;
+  p.ipb = (p.nd2 + pack - 1) / pack;
+  p.nsb = p.nb1 * p.ipb;
   p.RT1 = rt;
   p.wpr1 = wpr;
   p.ntiles1 = (uint32_t)((p.nrec + rt - 1) / rt);
   p.tcap1 = rt * p.S;
-  const double per_tile_bucket = (double)p.nmax / p.ntiles1 / p.nb1;
+  // canonical k-mers (min of a k-mer and its reverse complement) are twice as
+  // dense as the mean at the low end of the key range: size groups for that
+  const double per_tile_bucket = (double)p.nmax / p.ntiles1 / p.nb1 * (p.canonical ? 2.0 : 1.0);
   uint32_t G = (uint32_t)std::max(1.0, std::floor(0.75 * (double)cap2 / std::max(per_tile_bucket, 1e-9)));
   G = std::min<uint32_t>(G, L2_NT);
   G = std::min<uint32_t>(G, p.ntiles1);
@@ -435,7 +451,7 @@ void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<b
   b.flags_off = 0;
   for (size_t j = 0; j < nj; ++j) {
     b.off_flags[j] = o;
-    o += (size_t)plans[j].nsb * 8;
+    o += (size_t)plans[j].nb1 * plans[j].nd2 * 8;  // final look-back: one word per second-level digit
   }
   b.flags_words = o / 8;
   o = al256(o);
@@ -526,6 +542,8 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     m.ngroups = p.ngroups;
     m.cap2 = p.cap2;
     m.fin_nt = p.fin_nt;
+    m.pack = p.pack;
+    m.ipb = p.ipb;
     m.off1 = (uint16_t *)(ar + b.off_off1[j]);
     m.off2 = (uint16_t *)(ar + b.off_off2[j]);
     D.out1 = ar + b.off_out1[j];

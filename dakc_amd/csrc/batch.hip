@@ -345,7 +345,7 @@ extern "C" int dakc_batch_destroy(dakc_batch *b) {
 // -------------------------------------------------------------- autotuner --
 
 // Time the candidate MSD geometries (final-level workgroup of 256, 512 or 1024
-// threads, target sub-bucket fill 50/60/70%) and the LSD path on `job`, keep
+// threads, target final-item fill 60/75/90%) and the LSD path on `job`, keep
 // the fastest for the job's (key width, size class).  The count is exact for
 // every candidate; only the time differs.
 extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, double *best_ms) {
@@ -359,8 +359,8 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
   };
   // the default geometry first: another candidate replaces the best so far
   // only when it is more than 5% faster (the median of `trials` timed runs)
-  const Cand cands[] = {{1024, 60, 0}, {1024, 50, 0}, {1024, 70, 0}, {512, 50, 0},
-                        {512, 60, 0},  {512, 70, 0},  {256, 60, 0},  {1024, 60, 1}};
+  const Cand cands[] = {{1024, 90, 0}, {1024, 75, 0}, {1024, 60, 0}, {512, 90, 0},
+                        {512, 75, 0},  {512, 60, 0},  {256, 90, 0},  {1024, 90, 1}};
   TuneEntry best;
   best.ms = 1e300;
   dakc_table t;
@@ -436,7 +436,8 @@ extern "C" int dakc_tuning_load(dakc_ctx *ctx, const char *path) {
                     &kw, &cls, &fin, &fill, algo, &ms) == 6) {
       TuneEntry e;
       e.fin_nt = fin == 256 ? 256 : fin == 1024 ? 1024 : 512;
-      e.fill_pct = std::min(90u, std::max(30u, fill));
+      // above 100 the items outgrow LDS and are counted digit by digit
+      e.fill_pct = std::min(400u, std::max(30u, fill));
       e.algo = std::strcmp(algo, "lsd") == 0 ? 1 : 0;
       e.ms = ms;
       ctx->tuning[{kw, cls}] = e;

@@ -59,12 +59,14 @@ struct Plan {
   int ndig = 0, owner_digit = 0, npasses = 0;
   uint32_t canonical = 0, last_has_nl = 1;
   // MSD path (msd = 1): first-level bits b1 (with the owner when all bins
-  // are counted), second-level b2, in-LDS b3; L1 tiles of RT1 records
+  // are counted), second-level b2, in-LDS b3; L1 tiles of RT1 records.  A
+  // final item is `pack` consecutive second-level digits of a bucket (ipb
+  // items per bucket, nsb items in all).
   int msd = 0;
-  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1;
+  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1, pack = 1, ipb = 1;
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
   // tunables set before plan_choose_path (autotuner; defaults otherwise)
-  uint32_t tune_fin_nt = 1024, tune_fill_pct = 60;
+  uint32_t tune_fin_nt = 1024, tune_fill_pct = 90;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
@@ -162,7 +164,7 @@ struct ServeState;  // served mode (serve.hip)
 namespace dk {
 // autotuned MSD parameters for one (key width, job-size class)
 struct TuneEntry {
-  uint32_t fin_nt = 1024, fill_pct = 60, algo = 0;  // algo: 0 MSD, 1 LSD
+  uint32_t fin_nt = 1024, fill_pct = 90, algo = 0;  // algo: 0 MSD, 1 LSD
   double ms = 0;
 };
 int size_class(uint64_t nmax);

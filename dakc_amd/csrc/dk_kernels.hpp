@@ -121,6 +121,8 @@ struct MsdArgs {
   uint32_t ngroups;  // L2 groups per bucket
   uint32_t cap2;     // keys capacity of one (bucket, group)
   uint32_t fin_nt;   // threads of a final-level workgroup (256, 512 or 1024)
+  uint32_t pack;     // second-level digits per final item
+  uint32_t ipb;      // final items per bucket: ceil(nd2 / pack)
   uint16_t *off1;    // [nb1 + 1][ntiles1]
   uint16_t *off2;    // [nb1][nd2 + 1][ngroups]
 };

@@ -1163,6 +1163,25 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
 // Publish sub-bucket sb's distinct count H and return its exclusive prefix
 // over the earlier sub-buckets (decoupled look-back by wavefront 0).  Called
 // by the whole workgroup; ends with a barrier.
+// A final item: bucket b, second-level digits [s0, s0 + cnt) of it.  Its
+// keys go to LDS bin (digit - s0) << b3 | the next b3 bits, which orders
+// them as the keys order.
+struct FinItem {
+  uint32_t b, s0, cnt, nbins;
+};
+DK_DEV FinItem fin_item_of(const MsdArgs &m, uint32_t it) {
+  FinItem f;
+  f.b = it / m.ipb;
+  f.s0 = (it - f.b * m.ipb) * m.pack;
+  f.cnt = m.nd2 - f.s0 < m.pack ? m.nd2 - f.s0 : m.pack;
+  f.nbins = f.cnt << m.d3.nbits;
+  return f;
+}
+template <typename KeyT>
+DK_DEV uint32_t fin_bin(const MsdArgs &m, const FinItem &f, const KeyT &x) {
+  return ((key_bits(x, m.d2.shift, m.d2.nbits) - f.s0) << m.d3.nbits) | key_bits(x, m.d3.shift, m.d3.nbits);
+}
+
 DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t H, uint64_t *n_dev,
                            uint64_t *s_excl) {
   constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
@@ -1196,7 +1215,7 @@ template <int NT, int ITEMS, int HT>
 DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&keys)[ITEMS],
                          uint32_t mine, uint64_t *lds, uint32_t *s_h, uint32_t *s_wave,
                          uint32_t *s_total, uint32_t *s_fail, uint64_t *s_excl, uint32_t sb,
-                         uint32_t nsb) {
+                         uint32_t nsb, const FinItem &fi) {
   constexpr int SPT = HT / NT;  // table slots per thread
   const uint32_t tid = opaque_tid();
   uint64_t *tk = lds;                                     // HT keys, then the ordered keys
@@ -1244,7 +1263,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     return false;
   }
   // split the occupied slots by the next b3 bits
-  const uint32_t nd3 = 1u << m.d3.nbits;
+  const uint32_t nd3 = fi.nbins;
   uint64_t ek[SPT];
   uint32_t ec[SPT], ed[SPT], er[SPT];
 #pragma unroll
@@ -1252,7 +1271,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     ek[j] = tk[tid + j * NT];
     ec[j] = tc[tid + j * NT];
     if (ek[j] != HASH_EMPTY) {
-      ed[j] = key_bits(ek[j], m.d3.shift, m.d3.nbits);
+      ed[j] = fin_bin(m, fi, ek[j]);
       er[j] = atomicAdd(&s_h[ed[j]], 1u);
     }
   }
@@ -1333,17 +1352,18 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
   // group rows of sub-bucket sb (loads left in flight)
-  auto rows_of = [&](const MsdArgs &m, uint32_t sb, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT]) {
-    const uint32_t b = sb / m.nd2, s = sb - b * m.nd2;
-    const uint16_t *row0 = m.off2 + ((uint64_t)b * (m.nd2 + 1) + s) * m.ngroups;
-    load_rows<TPT>(row0, row0 + m.ngroups, m.ngroups, r0, r1);
+  auto rows_of = [&](const MsdArgs &m, const FinItem &f, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT]) {
+    const uint16_t *row0 = m.off2 + ((uint64_t)f.b * (m.nd2 + 1) + f.s0) * m.ngroups;
+    load_rows<TPT>(row0, row0 + (uint64_t)f.cnt * m.ngroups, m.ngroups, r0, r1);
   };
 
-  // One sub-bucket sb of job J, its group rows in r0/r1 and the job's
+  // One final item fi of job J, its group rows in r0/r1 and the job's
   // overflow counters in fo (tid 0).  `between` runs right after the gather,
-  // when the rows are free again; every thread returns together.
-  auto run = [&](const JobDesc &J, uint32_t sb, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT], uint64_t fo,
-                 auto &&between) {
+  // when the rows are free again; every thread returns together.  Returns
+  // true, before `between`, when the item holds more keys than LDS and more
+  // than one digit: the caller then runs its digits one by one.
+  auto run = [&](const JobDesc &J, const FinItem &fi, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT], uint64_t fo,
+                 auto &&between) -> bool {
     const uint32_t tid = opaque_tid();  // per item: no address is kept across items
     // distinct-key path: skip it once most earlier sub-buckets of the job
     // overflowed the table
@@ -1354,28 +1374,33 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     uint32_t *out_counts = J.tcounts;
     uint64_t *n_dev = J.nd;
     uint32_t *err = J.err;
-    const uint32_t nd3 = 1u << m.d3.nbits;
+    const uint32_t nd3 = fi.nbins;
     for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
     for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
     __syncthreads();
+    // look-back slots are digits: this item's count goes to its last digit
     const uint32_t nsb = m.nb1 * m.nd2;
-    const uint32_t b = sb / m.nd2;
+    const uint32_t sb = fi.b * m.nd2 + fi.s0 + fi.cnt - 1;
+    const uint32_t b = fi.b;
 
     KeyT keys[ITEMS];
     uint32_t mine;
     const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
         in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
         keys, mine);
+    if (tot > (uint32_t)CAP && fi.cnt > 1) return true;
     between();
     if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
     const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
+    // the item's inner digits add nothing to the prefix
+    if (tid + 1 < fi.cnt) lb_store(J.flags + sb + 1 - fi.cnt + tid, lb_pack(1, LB_AGG, 0));
 
     if constexpr (sizeof(KeyT) == 8) {
       static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
       if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
                                                          s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
-                                                         nsb))
-        return;
+                                                         nsb, fi))
+        return false;
     }
 
     // split by the next b3 bits into B (unstable, LDS atomics)
@@ -1383,7 +1408,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       if ((uint32_t)u < mine) {
-        dig[u] = key_bits(keys[u], m.d3.shift, m.d3.nbits);
+        dig[u] = fin_bin(m, fi, keys[u]);
         rank[u] = atomicAdd(&s_h[dig[u]], 1u);
       }
     }
@@ -1406,7 +1431,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if (i < mcnt) {
         const KeyT x = B[i];
         keys[q] = x;
-        dig[q] = key_bits(x, m.d3.shift, m.d3.nbits);
+        dig[q] = fin_bin(m, fi, x);
         if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
       }
     }
@@ -1466,6 +1491,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       out_keys[excl + h] = B[i];
       out_counts[excl + h] = end - i;
     }
+    return false;
   };
 
   if constexpr (ONE) {
@@ -1480,7 +1506,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     uint32_t r0[TPT], r1[TPT];
     uint64_t fo = 0;
     if (t < n) {
-      rows_of(one.m, t, r0, r1);
+      rows_of(one.m, fin_item_of(one.m, t), r0, r1);
       if (tid == 0) fo = fail_word(one);
     }
     while (t < n) {
@@ -1488,16 +1514,37 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if (tid == 0) nt = atomicAdd(a.ticket, 1u);  // its value is needed only after the gather
       uint32_t tn = n;
       uint64_t fon = 0;
-      run(one, t, r0, r1, fo, [&] {
+      auto fetch_next = [&] {
         if (tid == 0) s_next = nt;
         __syncthreads();
         tn = __builtin_amdgcn_readfirstlane(s_next);
         if (tn < n) {
-          rows_of(one.m, tn, r0, r1);
+          rows_of(one.m, fin_item_of(one.m, tn), r0, r1);
           if (tid == 0) fon = fail_word(one);
         }
-      });
-      __syncthreads();  // this sub-bucket's LDS is free, s_next read by all
+      };
+      // passes: the item, then, when it holds more keys than LDS, its
+      // digits one at a time (one call site, so one copy of the code)
+      const FinItem f = fin_item_of(one.m, t);
+      FinItem cur = f;
+      bool digits = false;
+      uint32_t d = 0;
+      for (;;) {
+        const bool split = run(one, cur, r0, r1, fo, [&] {
+          if (!digits) fetch_next();
+        });
+        if (split) digits = true;
+        else if (digits) ++d;
+        if (!digits) break;
+        __syncthreads();  // the previous pass is done with LDS
+        if (d == f.cnt) {
+          fetch_next();
+          break;
+        }
+        cur = FinItem{f.b, f.s0 + d, 1, 1u << one.m.d3.nbits};
+        rows_of(one.m, cur, r0, r1);
+      }
+      __syncthreads();  // this item's LDS is free, s_next read by all
       t = tn;
       fo = fon;
     }
@@ -1509,10 +1556,21 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     const uint32_t job = find_job_wg(a.pre3, a.njobs, item, &s_job);
     const JobDesc J = job_desc(a, job);
     uint32_t r0[TPT], r1[TPT];
-    const uint32_t sb = item - pre_at(a.pre3, job);
+    const FinItem f = fin_item_of(J.m, item - pre_at(a.pre3, job));
     const uint64_t fo = tid == 0 ? fail_word(J) : 0;
-    rows_of(J.m, sb, r0, r1);
-    run(J, sb, r0, r1, fo, [] {});
+    rows_of(J.m, f, r0, r1);
+    FinItem cur = f;
+    bool digits = false;  // more keys than LDS holds: then digit by digit
+    uint32_t d = 0;
+    for (;;) {
+      const bool split = run(J, cur, r0, r1, fo, [] {});
+      if (split) digits = true;
+      else if (digits) ++d;
+      if (!digits || d == f.cnt) break;
+      __syncthreads();
+      cur = FinItem{f.b, f.s0 + d, 1, 1u << J.m.d3.nbits};
+      rows_of(J.m, cur, r0, r1);
+    }
   }
 }
 

@@ -96,6 +96,8 @@ def test_genome_reads_repeats(kc):
     data = _lib.generate_reads(p)
     t = run_both(kc, data, 150, 31)
     assert t.counts.max() > 10
+    st = kc.last_stats()  # repeats still fit the default final items
+    assert st.path == (2 if kc.algo == _lib.ALGO_AUTO else 1) and st.msd_reruns == 0
     run_both(kc, data, 150, 31, canonical=True)
     run_both(kc, data, 150, 21)
 
@@ -209,3 +211,21 @@ def test_repeat_runs_identical(kc):
     for _ in range(3):
         b = kc.count(data)
         assert np.array_equal(a.keys, b.keys) and np.array_equal(a.counts, b.counts)
+
+
+def test_items_larger_than_lds_go_digit_by_digit(tmp_path):
+    """A tuned fill above 100% packs more second-level digits into a final
+    item than LDS holds: every item is then counted digit by digit in its
+    workgroup, still on the MSD path, with exact counts."""
+    p = _lib.gen_params(7, 200_000, 150, genome_len=1_000_000, sub_rate=0.005)
+    data = _lib.generate_reads(p)
+    path = tmp_path / "tune.json"
+    path.write_text('{\n "entries": [\n  {"key_words": 1, "size_class": 24, "fin_nt": 1024, '
+                    '"fill_pct": 150, "algo": "msd", "ms": 1.0}\n ]\n}\n')
+    with KmerCounter(k=31, read_len=150) as kc2:
+        kc2.tuning_load(path)
+        for canonical in (False, True):
+            g = kc2.count(data, canonical=canonical)
+            st = kc2.last_stats()
+            assert st.path == 2 and st.msd_reruns == 0
+            assert_same(g, oracle.count(data, 150, 31, canonical))

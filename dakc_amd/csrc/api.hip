@@ -345,7 +345,11 @@ int plan_choose_path(Plan *pl, int algo) {
   const uint32_t fin_nt = p.tune_fin_nt == 256 ? 256u : p.tune_fin_nt == 1024 ? 1024u : 512u;
   const uint64_t cap3 = (uint64_t)fin_cap(p.kw, (int)fin_nt), cap1 = (uint64_t)msd_tile(p.kw),
                  cap2 = (uint64_t)l2_cap(p.kw);
-  const uint64_t target = cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100;
+  // at most the tuned fill, and small enough for a final item per CU (a
+  // small job in few large items would leave most of the GPU idle)
+  const uint64_t target = std::min<uint64_t>(
+      cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100,
+      std::max<uint64_t>(cap3 / 8, p.nmax / std::max(p.num_cus, 1u)));
   uint32_t b12 = 0;
   while ((p.nmax >> b12) > target) ++b12;
   const uint32_t owners = p.owner_digit ? p.num_pes : 1;
@@ -599,6 +603,7 @@ static BatchArgs batch_args(const JobWs &ws, const MsdBatch &b) {
   a.njobs = (uint32_t)nj;
   a.ticket = pre + 3 * (nj + 1);
   a.item0 = 0;
+  a.persist = ws.fin_persistent ? 1u : 0u;
   return a;
 }
 

@@ -34,6 +34,7 @@ void apply_tuning(const dakc_ctx *c, Plan *pl, int algo) {
   }
   pl->tune_fin_nt = e.fin_nt;
   pl->tune_fill_pct = e.fill_pct;
+  pl->num_cus = (uint32_t)std::max(c->num_cus, 1);
   plan_choose_path(pl, algo == DAKC_ALGO_AUTO && e.algo == 1 ? DAKC_ALGO_LSD : algo);
 }
 

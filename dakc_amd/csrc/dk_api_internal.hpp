@@ -65,8 +65,9 @@ struct Plan {
   int msd = 0;
   uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1, pack = 1, ipb = 1;
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
-  // tunables set before plan_choose_path (autotuner; defaults otherwise)
-  uint32_t tune_fin_nt = 1024, tune_fill_pct = 90;
+  // tunables set before plan_choose_path (autotuner; defaults otherwise),
+  // and the CUs of the device the job runs on
+  uint32_t tune_fin_nt = 1024, tune_fill_pct = 90, num_cus = 256;
 };
 
 // internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
@@ -114,6 +115,10 @@ struct JobWs {
       msd_off1, msd_off2, arena, meta, desc;
   TableLoc loc;     // table of the last enqueued count
   MsdBatch single;  // layout of the last single-job MSD count
+  // one-job final levels: persistent workgroups over all the items (plain
+  // counts), or one item per workgroup (served slots, whose jobs share the
+  // GPU: short workgroups let the other slots' kernels in between)
+  bool fin_persistent = true;
   hipStream_t stream = nullptr;
   hipEvent_t ev[MAX_PASSES + 4] = {};
   uint32_t epoch = 0;

@@ -147,6 +147,7 @@ struct BatchArgs {
   uint32_t *ticket;                    // final-level ticket of this launch
   uint32_t item0;                      // first final-level item of this launch
   uint32_t nfin;                       // final-level items of this launch
+  uint32_t persist;                    // one-job final level: 1 persistent workgroups, 0 one item each
 };
 // launch the MSD levels for every job of `a` (n1 L1 items, n2 L2 items, n3
 // final items from a.item0 on; a zero count skips the level).  kw is shared

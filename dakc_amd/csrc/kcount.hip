@@ -1510,8 +1510,8 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if (tid == 0) fo = fail_word(one);
     }
     while (t < n) {
-      uint32_t nt = 0;
-      if (tid == 0) nt = atomicAdd(a.ticket, 1u);  // its value is needed only after the gather
+      uint32_t nt = n;  // not persistent: this workgroup's one item only
+      if (tid == 0 && a.persist) nt = atomicAdd(a.ticket, 1u);  // its value is needed only after the gather
       uint32_t tn = n;
       uint64_t fon = 0;
       auto fetch_next = [&] {
@@ -1680,9 +1680,11 @@ static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint
   if (n3) {
     // one job: persistent workgroups (as many as fit beside each other, by
     // thread count) over the tickets; several jobs: one workgroup per item
+    // (two-word keys measured faster one item per workgroup)
     BatchArgs af = a;
     af.nfin = n3;
-    const uint32_t grid = ONE ? std::min(n3, device_cus(st) * (uint32_t)(1024 / NT3)) : n3;
+    af.persist = a.persist && sizeof(KeyT) == 8 ? 1u : 0u;
+    const uint32_t grid = ONE && af.persist ? std::min(n3, device_cus(st) * (uint32_t)(1024 / NT3)) : n3;
     k_fin<KeyT, NT3, FI, NB3, HT, ONE><<<grid, NT3, 0, st>>>(af, one);
   }
 }

@@ -407,6 +407,7 @@ static ServeState *serve_get(dakc_ctx *c, int *status) {
       delete s;
       return nullptr;
     }
+    sl->ws.fin_persistent = false;  // slots share the GPU: short final workgroups
     s->slots.push_back(std::move(sl));
   }
   s->sched = std::thread(scheduler, s);

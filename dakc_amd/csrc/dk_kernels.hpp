@@ -108,7 +108,7 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
 //   k_l1  records -> k-mers in first-level bucket order, per tile, with the
 //         tile's bucket starts (off1[bucket][tile])
 //   k_l2  bucket b, group of G consecutive L1 tiles -> keys in second-level
-//         digit order, with the digit starts (off2[b][digit][group])
+//         digit order, with the digit starts (off2[b][group][digit])
 //   k_fin sub-bucket (b, digit) gathered from b's groups into LDS, sorted,
 //         run-length reduced, written in place by the decoupled look-back
 struct MsdArgs {
@@ -124,7 +124,7 @@ struct MsdArgs {
   uint32_t pack;     // second-level digits per final item
   uint32_t ipb;      // final items per bucket: ceil(nd2 / pack)
   uint16_t *off1;    // [nb1 + 1][ntiles1]
-  uint16_t *off2;    // [nb1][nd2 + 1][ngroups]
+  uint16_t *off2;    // [nb1][ngroups][nd2 + 1]
 };
 
 // One job of a multi-job MSD launch.  Work items of every stage are numbered

@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
 
 // L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
 // digit order, written at (b * ngroups + g) * cap2, with the digit starts in
-// off2[b][digit][g].  Items are numbered g * nb1 + b.  Persistent like k_l1:
+// off2[b][g][digit].  Items are numbered g * nb1 + b.  Persistent like k_l1:
 // each XCD owns a contiguous range of items, so the workgroups of an XCD take
 // consecutive buckets of one group, whose segments are neighbours in every L1
 // tile (the cache lines they share stay in that XCD's L2).  The next item's
@@ -1097,9 +1097,10 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     if (nitem < hi) tn = gather_item(nitem, cn, vn, mn);
     else __syncthreads();
     scan_lds<NT>(s_h, m.nd2, s_wave, &s_total);
-    uint16_t *off = m.off2 + (uint64_t)b * (m.nd2 + 1) * m.ngroups + g;
-    if (tid <= m.nd2) off[(uint64_t)tid * m.ngroups] = (uint16_t)(tot ? s_h[tid] : 0);
-    if (tid == 0 && m.nd2 >= NT) off[(uint64_t)NT * m.ngroups] = (uint16_t)(tot ? s_h[NT] : 0);
+    // this item's digit starts, one contiguous row of off2 [bucket][group][digit]
+    uint16_t *off = m.off2 + ((uint64_t)b * m.ngroups + g) * (m.nd2 + 1);
+    if (tid <= m.nd2) off[tid] = (uint16_t)(tot ? s_h[tid] : 0);
+    if (tid == 0 && m.nd2 >= NT) off[NT] = (uint16_t)(tot ? s_h[NT] : 0);
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
       if ((uint32_t)u < mine) stage[s_h[dr[u] >> 16] + (dr[u] & 0xFFFFu)] = v[u];
@@ -1353,8 +1354,20 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   };
   // group rows of sub-bucket sb (loads left in flight)
   auto rows_of = [&](const MsdArgs &m, const FinItem &f, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT]) {
-    const uint16_t *row0 = m.off2 + ((uint64_t)f.b * (m.nd2 + 1) + f.s0) * m.ngroups;
-    load_rows<TPT>(row0, row0 + (uint64_t)f.cnt * m.ngroups, m.ngroups, r0, r1);
+    // digit starts s0 and s0 + cnt of every group of the bucket (rows of the
+    // bucket's groups lie together, so the items of a bucket share them in L2)
+    const uint16_t *base = m.off2 + (uint64_t)f.b * m.ngroups * (m.nd2 + 1);
+    const uint32_t tid = opaque_tid();
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const uint32_t q = tid * TPT + u;
+      r0[u] = r1[u] = 0;
+      if (q < m.ngroups) {
+        const uint16_t *row = base + (uint64_t)q * (m.nd2 + 1);
+        r0[u] = row[f.s0];
+        r1[u] = row[f.s0 + f.cnt];
+      }
+    }
   };
 
   // One final item fi of job J, its group rows in r0/r1 and the job's

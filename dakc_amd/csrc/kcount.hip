@@ -1208,8 +1208,9 @@ constexpr uint64_t HASH_EMPTY = ~0ull;  // no k-mer of k < 32 bases has all 64 b
 // Distinct-key path of the final level (keys of k < 32 bases): the gathered
 // keys go into an LDS hash table (equal keys add to one count), the occupied
 // slots are split by the next b3 bits, the few distinct keys of each bin are
-// ordered by counting, and the (key, count) pairs are written out.  The table
-// is HT slots of key + count; the ordered pairs later take its place.
+// ranked by counting, and each (key, count) pair goes from registers to its
+// place in the output.  The table is HT slots of key + count; the binned
+// pairs later take its place.
 // Returns false, having changed nothing the other path needs, when a key
 // finds no slot within 32 probes.  Uniform over the workgroup.
 template <int NT, int ITEMS, int HT>
@@ -1263,67 +1264,74 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     __syncthreads();  // everyone has read s_fail before the other path reuses LDS
     return false;
   }
-  // split the occupied slots by the next b3 bits
+  // split the occupied slots by the next b3 bits; counts move with their keys
   const uint32_t nd3 = fi.nbins;
-  uint64_t ek[SPT];
-  uint32_t ec[SPT], ed[SPT], er[SPT];
+  {
+    uint64_t ek[SPT];
+    uint32_t ec[SPT], ed[SPT], er[SPT];
 #pragma unroll
-  for (int j = 0; j < SPT; ++j) {
-    ek[j] = tk[tid + j * NT];
-    ec[j] = tc[tid + j * NT];
-    if (ek[j] != HASH_EMPTY) {
-      ed[j] = fin_bin(m, fi, ek[j]);
-      er[j] = atomicAdd(&s_h[ed[j]], 1u);
+    for (int j = 0; j < SPT; ++j) {
+      ek[j] = tk[tid + j * NT];
+      ec[j] = tc[tid + j * NT];
+      if (ek[j] != HASH_EMPTY) {
+        ed[j] = fin_bin(m, fi, ek[j]);
+        er[j] = atomicAdd(&s_h[ed[j]], 1u);
+      }
     }
+    __syncthreads();
+    scan_lds<NT>(s_h, nd3, s_wave, s_total);
+#pragma unroll
+    for (int j = 0; j < SPT; ++j)
+      if (ek[j] != HASH_EMPTY) {
+        const uint32_t p = s_h[ed[j]] + er[j];
+        tk[p] = ek[j];
+        tc[p] = ec[j];
+      }
   }
-  __syncthreads();
-  scan_lds<NT>(s_h, nd3, s_wave, s_total);
   const uint32_t H = s_h[nd3];
   constexpr uint32_t epoch = 1;
   if (tid == 0) lb_store(J.flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));  // early
-#pragma unroll
-  for (int j = 0; j < SPT; ++j)
-    if (ek[j] != HASH_EMPTY) {
-      tk[s_h[ed[j]] + er[j]] = ek[j];
-      er[j] = s_h[ed[j]];  // bin start
-    }
   __syncthreads();
-  // order the distinct keys of each bin: place = bin start + number smaller
+  // order the distinct keys of each bin, positions tid, tid + NT, ... < H:
+  // place = bin start + number of smaller keys in the bin
+  uint64_t x[SPT];
+  uint32_t xc[SPT], place[SPT];
   bool big = false;
 #pragma unroll
-  for (int j = 0; j < SPT; ++j) {
-    if (ek[j] == HASH_EMPTY) continue;
-    const uint32_t lo = er[j], hi = s_h[ed[j] + 1];
-    uint32_t r = 0;
-    if (hi - lo > 1) {
-      if (hi - lo > 256) {
-        big = true;
-      } else {
-        for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
-          uint64_t y[8];
+  for (int i = 0; i < SPT; ++i) {
+    const uint32_t p = tid + i * NT;
+    if (p < H) {
+      x[i] = tk[p];
+      xc[i] = tc[p];
+      const uint32_t d = fin_bin(m, fi, x[i]);
+      const uint32_t lo = s_h[d], hi = s_h[d + 1];
+      uint32_t r = 0;
+      if (hi - lo > 1) {
+        if (hi - lo > 256) {
+          big = true;
+        } else {
+          for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+            uint64_t y[8];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) y[q] = t0 + q < hi ? tk[t0 + q] : ek[j];
+            for (int q = 0; q < 8; ++q) y[q] = t0 + q < hi ? tk[t0 + q] : x[i];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) r += y[q] < ek[j];
+            for (int q = 0; q < 8; ++q) r += y[q] < x[i];
+          }
         }
       }
+      place[i] = lo + r;
     }
-    er[j] = lo + r;
   }
   if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < SPT; ++j)
-    if (ek[j] != HASH_EMPTY) {
-      tk[er[j]] = ek[j];
-      tc[er[j]] = ec[j];
-    }
+  // the bins are contiguous, so a wave's 64 pairs land in a few lines
   const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, s_excl);
   uint64_t *out_keys = reinterpret_cast<uint64_t *>(J.out1);
-  for (uint32_t h = tid; h < H; h += NT) {
-    out_keys[excl + h] = tk[h];
-    J.tcounts[excl + h] = tc[h];
-  }
+#pragma unroll
+  for (int i = 0; i < SPT; ++i)
+    if (tid + i * NT < H) {
+      out_keys[excl + place[i]] = x[i];
+      J.tcounts[excl + place[i]] = xc[i];
+    }
   return true;
 }
 

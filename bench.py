@@ -1,6 +1,6 @@
 """Benchmark of the k-mer counting hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|iid]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|genome_k55|iid]
 
 A step is one full count (decode, extract, radix sort, run-length reduce) of
 the genome-like workload: a 100 Mbp uniform-ACGT genome, 20,000,000 reads of
@@ -33,6 +33,9 @@ WORKLOADS = {
     # genome-like single job: 1e8 bp genome, 30x of 150 bp reads, 0.5% errors, k=31
     "genome": dict(seed=2025, num_reads=20_000_000, read_len=150, genome_len=100_000_000,
                    sub_rate=0.005, k=31),
+    # the same reads at k=55: two-word keys (1.92e9 k-mers)
+    "genome_k55": dict(seed=2025, num_reads=20_000_000, read_len=150, genome_len=100_000_000,
+                       sub_rate=0.005, k=55),
     # plain single job: 1M i.i.d. reads x 150, canonical k=31
     "iid": dict(seed=2024, num_reads=1_000_000, read_len=150, genome_len=0, sub_rate=0.0, k=31,
                 canonical=True),

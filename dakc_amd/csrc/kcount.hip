@@ -44,6 +44,20 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
     p[i] = v;
 }
 
+// Sum of the first `below` of NW per-wave values s[0, NW) (NW a power of two
+// <= 64), in every lane: each lane reads one value and a butterfly of NW
+// lanes adds them; one LDS read and log2(NW) shuffles instead of a chain of
+// dependent reads.  Every lane must be active.
+template <int NW, typename T>
+DK_DEV T sum_waves_below(const T *s, uint32_t below, uint32_t lane) {
+  static_assert(NW >= 1 && NW <= 64 && (NW & (NW - 1)) == 0, "a power of two of at most 64");
+  const uint32_t l = lane & (NW - 1);
+  T t = l < below ? s[l] : T(0);
+#pragma unroll
+  for (int o = NW / 2; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+  return t;
+}
+
 // Exclusive scan of 256 u32 values held by threads [0, 256); all threads call.
 // Returns the exclusive prefix for threads < 256; total in *s_total.
 template <int NT>
@@ -58,16 +72,16 @@ DK_DEV uint32_t scan256(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
   }
   if (tid < 256 && lane == 63) s_wave[wave] = x;
   __syncthreads();
-  uint32_t pre = 0;
-  if (tid < 256)
-    for (uint32_t w = 0; w < wave; ++w) pre += s_wave[w];
+  const uint32_t pre = sum_waves_below<4>(s_wave, wave, lane);
   if (tid == 255) *s_total = pre + x;
   __syncthreads();
   return pre + x - (tid < 256 ? v : 0);
 }
 
-// Exclusive scan over all NT threads of one u32 each.
-template <int NT>
+// Exclusive scan over all NT threads of one u32 each.  SHFL: the earlier
+// waves' totals are added by shuffles (one LDS read); otherwise by a chain of
+// LDS reads, which needs fewer registers in a kernel at its register limit.
+template <int NT, bool SHFL = true>
 DK_DEV uint32_t scan_block(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
   const uint32_t tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   uint32_t x = v;
@@ -79,7 +93,11 @@ DK_DEV uint32_t scan_block(uint32_t v, uint32_t *s_wave, uint32_t *s_total) {
   if (lane == 63) s_wave[wave] = x;
   __syncthreads();
   uint32_t pre = 0;
-  for (uint32_t w = 0; w < wave; ++w) pre += s_wave[w];
+  if constexpr (SHFL) {
+    pre = sum_waves_below<NT / 64>(s_wave, wave, lane);
+  } else {
+    for (uint32_t w = 0; w < wave; ++w) pre += s_wave[w];
+  }
   if (tid == NT - 1) *s_total = pre + x;
   __syncthreads();
   return pre + x - v;
@@ -532,8 +550,9 @@ __global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uin
 // bucket into LDS, sorts it there, run-length reduces it and writes its
 // (kmer, count) entries at the position the decoupled look-back gives.
 
-// exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = total
-template <int NT>
+// exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = total (SHFL as
+// in scan_block)
+template <int NT, bool SHFL = true>
 DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_total) {
   const uint32_t tid = opaque_tid();
   uint32_t v[4], sum = 0;
@@ -543,7 +562,7 @@ DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_tota
     v[q] = i < n ? s[i] : 0;
     sum += v[q];
   }
-  uint32_t pre = scan_block<NT>(sum, s_wave, s_total);
+  uint32_t pre = scan_block<NT, SHFL>(sum, s_wave, s_total);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const uint32_t i = tid * 4 + q;
@@ -597,7 +616,8 @@ DK_DEV void load_rows(const uint16_t *row0, const uint16_t *row1, uint32_t nseg,
   }
 }
 
-template <typename KeyT, int NT, int ITEMS, int TPT, int NSEG>
+// SHFL as in scan_block.
+template <typename KeyT, int NT, int ITEMS, int TPT, int NSEG, bool SHFL>
 DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], const uint32_t (&r1)[TPT],
                                 uint32_t nseg, uint64_t region0, uint32_t stride,
                                 GatherSmem<NSEG, NT * ITEMS> &g, uint32_t *s_wave,
@@ -631,7 +651,11 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], cons
   if (lane == 63) s_w64[tid >> 6] = x;
   __syncthreads();
   uint64_t pre = 0;
-  for (uint32_t w = 0; w < (tid >> 6); ++w) pre += s_w64[w];
+  if constexpr (SHFL) {
+    pre = sum_waves_below<NT / 64>(s_w64, tid >> 6, lane);
+  } else {
+    for (uint32_t w = 0; w < (tid >> 6); ++w) pre += s_w64[w];
+  }
   if (tid == NT - 1) *s_total = (uint32_t)(pre + x);
   pre += x - (((uint64_t)cnt << 32) | sum);
   __syncthreads();
@@ -729,6 +753,41 @@ DK_DEV void cursor_seek(JobCursor &c, const uint32_t *pre, uint32_t njobs, uint3
   c.job = find_job_wg(pre, njobs, it, s_job);
   c.lo = pre_at(pre, c.job);
   c.hi = pre_at(pre, c.job + 1);
+}
+
+// Staged keys [0, tot) of a workgroup's LDS to o[0, tot), thread t taking
+// positions t + u * NT (coalesced): a chunk of CH LDS reads, then its
+// stores.  Reads stay in range (positions < NT * ITEMS) and are not stored
+// past the total.  The chunk is held as 64-bit words, which stay in
+// registers for two-word keys too (an array of key structs went to scratch).
+template <typename KeyT, int NT, int ITEMS>
+DK_DEV void write_staged(const KeyT *stage, KeyT *o, uint32_t tot, uint32_t tid) {
+  constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;
+  constexpr int W = sizeof(KeyT) / 8;
+  static_assert(W == 1 || W == 2, "one- or two-word keys");
+#pragma unroll
+  for (int c0 = 0; c0 < ITEMS; c0 += CH) {
+    uint64_t w[CH * W];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (c0 + i < ITEMS) {
+        const KeyT x = stage[tid + (c0 + i) * NT];
+        if constexpr (W == 1) {
+          w[i] = x;
+        } else {
+          w[2 * i] = x.lo;
+          w[2 * i + 1] = x.hi;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (c0 + i < ITEMS && tid + (c0 + i) * NT < tot) {
+        if constexpr (W == 1) o[tid + (c0 + i) * NT] = w[i];
+        else o[tid + (c0 + i) * NT] = KeyT{w[2 * i], w[2 * i + 1]};
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------- L1 extraction --
@@ -1004,18 +1063,7 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
     for (int j = 0; j < ITEMS; ++j)
       if ((vmask >> j) & 1) sm.stage[s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = keys[j];
     __syncthreads();
-    KeyT *o = out1 + (uint64_t)tile * m.tcap1;
-    constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;  // LDS reads of a chunk, then its stores
-#pragma unroll
-    for (int c0 = 0; c0 < ITEMS; c0 += CH) {
-      KeyT w[CH];
-#pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (c0 + i < ITEMS && tid + (c0 + i) * NT < total) w[i] = sm.stage[tid + (c0 + i) * NT];
-#pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (c0 + i < ITEMS && tid + (c0 + i) * NT < total) o[tid + (c0 + i) * NT] = w[i];
-    }
+    write_staged<KeyT, NT, ITEMS>(sm.stage, out1 + (uint64_t)tile * m.tcap1, total, tid);
   }
 }
 
@@ -1055,7 +1103,7 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     const uint16_t *row0 = mg.off1 + (uint64_t)bb * mg.ntiles1 + t0;
     uint32_t r0[1], r1[1];
     load_rows<1>(row0, row0 + mg.ntiles1, nt, r0, r1);
-    uint32_t t = gather_segments<KeyT, NT, ITEMS, 1, NT>(
+    uint32_t t = gather_segments<KeyT, NT, ITEMS, 1, NT, true>(
         reinterpret_cast<const KeyT *>(Jg.out1), r0, r1, nt,
         (uint64_t)t0 * mg.tcap1, mg.tcap1, s_g, s_gwave, &s_gtotal, dst, mn);
     if (t > (uint32_t)CAP) {
@@ -1108,18 +1156,7 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     // the next item's keys have arrived: settle them before this item's
     // stores, so the next iteration waits for nothing
     __builtin_amdgcn_s_waitcnt(VMCNT0);
-    KeyT *o = out2 + ((uint64_t)b * m.ngroups + g) * m.cap2;
-    constexpr int CH = ITEMS % 5 == 0 ? 5 : 4;
-#pragma unroll
-    for (int c0 = 0; c0 < ITEMS; c0 += CH) {
-      KeyT w[CH];
-#pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (c0 + i < ITEMS && tid + (c0 + i) * NT < tot) w[i] = stage[tid + (c0 + i) * NT];
-#pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (c0 + i < ITEMS && tid + (c0 + i) * NT < tot) o[tid + (c0 + i) * NT] = w[i];
-    }
+    write_staged<KeyT, NT, ITEMS>(stage, out2 + ((uint64_t)b * m.ngroups + g) * m.cap2, tot, tid);
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) v[u] = vn[u];
     cur = cn;
@@ -1279,7 +1316,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
       }
     }
     __syncthreads();
-    scan_lds<NT>(s_h, nd3, s_wave, s_total);
+    scan_lds<NT, false>(s_h, nd3, s_wave, s_total);  // at the register limit
 #pragma unroll
     for (int j = 0; j < SPT; ++j)
       if (ek[j] != HASH_EMPTY) {
@@ -1406,7 +1443,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
 
     KeyT keys[ITEMS];
     uint32_t mine;
-    const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS>(
+    const uint32_t tot = gather_segments<KeyT, NT, ITEMS, TPT, FIN_MAX_GROUPS, false>(
         in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
         keys, mine);
     if (tot > (uint32_t)CAP && fi.cnt > 1) return true;
@@ -1434,7 +1471,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       }
     }
     __syncthreads();
-    scan_lds<NT>(s_h, nd3, s_wave, &s_total);
+    scan_lds<NT, false>(s_h, nd3, s_wave, &s_total);  // at the register limit
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
       if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
@@ -1499,7 +1536,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       hmask |= (h ? 1u : 0u) << q;
       heads += h;
     }
-    uint32_t o = scan_block<NT>(heads, s_wave, &s_total);
+    uint32_t o = scan_block<NT, false>(heads, s_wave, &s_total);
     const uint32_t H = s_total;
     uint16_t *s_hp = s_u.hp;
   #pragma unroll

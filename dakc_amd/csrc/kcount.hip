@@ -798,6 +798,8 @@ DK_DEV void write_staged(const KeyT *stage, KeyT *o, uint32_t tot, uint32_t tid)
 // is the previous one shifted by one base (kcounter_funcs.cpp:25-38 does the
 // same per read).
 
+constexpr int L1_RAWW = L1_RAW / 4 + 4;  // LDS words of an L1 tile's raw records (+ alignment)
+
 // Geometry of a tile's raw bytes: records r0 .. r0 + nr - 1 of the job start
 // at byte b0; word i of the tile is the 4 bytes at a0 + 4i, a0 = b0 rounded
 // down to 4, nwords of them.
@@ -896,19 +898,24 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
     if (w < wpl) {
       const uint8_t *rec = raw + (uint64_t)r * rd.stride;
       const uint32_t j0 = w * 32;
+      // bytes j0 .. j0 + 31 of the record: nine aligned LDS words (a word
+      // past the buffer only holds bytes past the read, which are masked),
+      // realigned and decoded four at a time
+      const uint32_t off = (uint32_t)(b0 - t.a0) + r * rd.stride + j0;
+      const uint32_t i0 = off >> 2, sh = 8 * (off & 3);
+      uint32_t dw[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) dw[q] = raw32[i0 + q < (uint32_t)L1_RAWW ? i0 + q : L1_RAWW - 1];
+      const uint32_t nvalid = rd.L - j0;  // bases of the read from j0 on (>= 1)
       uint32_t mloc = 0xFFFFFFFFu;
       inv = 0;
-#pragma unroll 8
-      for (uint32_t q = 0; q < 32; ++q) {
-        const uint32_t j = j0 + q;
-        const uint32_t b = j < rd.L ? rec[j] : 0u;
-        const uint32_t u = b & 0xDFu;  // upper-case letters
-        const bool ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T');
-        const uint32_t tt = (b >> 1) & 3u;          // A0 C1 T2 G3 (ASCII bits 1-2)
-        const uint32_t c = tt ^ ((tt >> 1) ^ 1u);  // -> C0 A1 T2 G3
-        code = (code << 2) | (ok ? c : 0u);
-        inv |= (ok ? 0u : 1u) << q;
-        if (u == 'M' && mloc == 0xFFFFFFFFu) mloc = j;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t x = (uint32_t)((((uint64_t)dw[q + 1] << 32) | dw[q]) >> sh);
+        const Dec4 d = decode4(x, nvalid > 4u * q ? nvalid - 4u * q : 0u);
+        code |= (uint64_t)d.code8 << (56 - 8 * q);
+        inv |= d.inv4 << (4 * q);
+        if (d.mmask && mloc == 0xFFFFFFFFu) mloc = j0 + 4 * q + (__builtin_ctz(d.mmask) >> 3);
       }
       if (mloc != 0xFFFFFFFFu) atomicMin(&sm.mpos[r], mloc);
       if (w == 0 && rec[0] == 0) atomicOr(err, ERR_NUL_RECORD);
@@ -981,7 +988,7 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
 template <typename KeyT, int NT, int ITEMS, bool ONE>
 __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   constexpr int TILE = NT * ITEMS;
-  constexpr int RAWW = L1_RAW / 4 + 4;  // raw words of a tile (records + alignment)
+  constexpr int RAWW = L1_RAWW;
   union Smem {
     L1Code ex;
     KeyT stage[TILE];
@@ -1738,7 +1745,8 @@ static void msd_levels(const BatchArgs &a, const JobDesc &one, uint32_t n1, uint
   if (n3) {
     // one job: persistent workgroups (as many as fit beside each other, by
     // thread count) over the tickets; several jobs: one workgroup per item
-    // (two-word keys measured faster one item per workgroup)
+    // (two-word keys run one item per workgroup: persistent, the k=55 genome
+    // count's final level measured 113 ms against 51 ms)
     BatchArgs af = a;
     af.nfin = n3;
     af.persist = a.persist && sizeof(KeyT) == 8 ? 1u : 0u;

@@ -75,6 +75,26 @@ def test_messy_reads_all_k(kc, k, canonical):
     run_both(kc, messy_reads(k * 7 + canonical, 700, L), L, k, canonical)
 
 
+@pytest.mark.parametrize("L", [150, 61])
+def test_every_byte_value(kc, L):
+    """Every byte value 1-255 inside reads of ACGT/acgt, at every offset within
+    a 4-byte word (records of odd width start at every alignment), singly and
+    in clumps: the decode treats each byte exactly as the oracle does."""
+    rng = np.random.default_rng(L)
+    n = 255 * 8
+    recs = np.frombuffer(b"ACGTacgt", np.uint8)[rng.integers(0, 8, size=(n, L))]
+    for i in range(n):
+        v = 1 + i % 255
+        p = (i // 255) + 4 * int(rng.integers(0, (L - 8) // 4))  # offsets 0..7 of a word pair
+        recs[i, p] = v
+        if i % 3 == 0:  # a clump of arbitrary bytes after it
+            q = int(rng.integers(1, 6))
+            recs[i, p + 1:min(L, p + 1 + q)] = rng.integers(1, 256, size=min(L, p + 1 + q) - p - 1)
+    data = np.concatenate([recs, np.full((n, 1), 10, np.uint8)], axis=1).tobytes()
+    for k in (1, 5, 21, 31):
+        run_both(kc, data, L, k)
+
+
 @pytest.mark.parametrize("L", [1, 7, 31, 32, 33, 63, 64, 65, 100, 257, 1000, 4096])
 def test_read_widths(kc, L):
     data = messy_reads(L, 200 if L < 1000 else 20, L)

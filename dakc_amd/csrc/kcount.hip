@@ -1281,7 +1281,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     if ((uint32_t)u < mine) {
-      hh[u] = (uint32_t)((keys[u] * 0x9E3779B97F4A7C15ull) >> 40) & (HT - 1);
+      hh[u] = (uint32_t)((((keys[u] * 0x9E3779B97F4A7C15ull) >> 32) * HT) >> 32);  // [0, HT)
       old[u] = atomicCAS((unsigned long long *)&tk[hh[u]], HASH_EMPTY, keys[u]);
     }
   }
@@ -1293,7 +1293,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
       uint32_t h = hh[u];
       uint64_t o = old[u];
       for (int pr = 0; pr < 32 && o != HASH_EMPTY && o != x; ++pr) {
-        h = (h + 1) & (HT - 1);
+        h = h + 1 == (uint32_t)HT ? 0u : h + 1;
         o = atomicCAS((unsigned long long *)&tk[h], HASH_EMPTY, x);
       }
       if (o == HASH_EMPTY || o == x) atomicAdd(&tc[h], 1u);
@@ -1759,9 +1759,10 @@ template <typename KeyT, int ITEMS, bool ONE>
 static void msd_levels_nt(uint32_t fin_nt, const BatchArgs &a, const JobDesc &one, uint32_t n1,
                           uint32_t n2, uint32_t n3, hipStream_t st) {
   constexpr bool W1 = sizeof(KeyT) == 8;
-  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 1024, W1 ? 8192 : 0, ONE>(a, one, n1, n2, n3, st);
-  else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 4096 : 0, ONE>(a, one, n1, n2, n3, st);
-  else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2048 : 0, ONE>(a, one, n1, n2, n3, st);
+  // hash slots (12 bytes each) filling the final level's key buffer
+  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 1024, W1 ? 10240 : 0, ONE>(a, one, n1, n2, n3, st);
+  else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 5120 : 0, ONE>(a, one, n1, n2, n3, st);
+  else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2560 : 0, ONE>(a, one, n1, n2, n3, st);
 }
 
 void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc *one, uint32_t n1,

@@ -370,7 +370,12 @@ int plan_choose_path(Plan *pl, int algo) {
   rt = std::max<uint32_t>(rt, 1);
   if ((uint64_t)rt * p.S > cap1) return 0;
   const uint32_t b2f = std::min(std::min(10u, bits - b1), b2 + 2);
-  const double per_digit = (double)p.nmax / (double)(1ull << (b1 + b2f));
+  // keys per digit where they are densest: canonical k-mers (the smaller of a
+  // k-mer and its reverse complement) are twice the mean at the low end of
+  // the key range.  An item over LDS capacity is redone digit by digit, and
+  // its later digits' look-back slots stay unpublished meanwhile, so every
+  // later item waits on it: items are sized to fit where keys are densest.
+  const double per_digit = (double)p.nmax / (double)(1ull << (b1 + b2f)) * (p.canonical ? 2.0 : 1.0);
   uint32_t pack = (uint32_t)std::max(1.0, std::floor((double)target / std::max(per_digit, 1e-9)));
   pack = std::min<uint32_t>(std::min<uint32_t>(pack, 16), 1u << b2f);
   const uint32_t nb3 = fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup

@@ -347,11 +347,14 @@ int plan_choose_path(Plan *pl, int algo) {
                  cap2 = (uint64_t)l2_cap(p.kw);
   // at most the tuned fill, and small enough for a final item per CU (a
   // small job in few large items would leave most of the GPU idle)
+  // keys expected to reach the sort: with one owner bin of P counted, the
+  // owner filter keeps about 1/P of the slots (MurmurHash64A spreads them)
+  const uint64_t nexp = p.owner_bin >= 0 && p.num_pes > 1 ? (p.nmax + p.num_pes - 1) / p.num_pes : p.nmax;
   const uint64_t target = std::min<uint64_t>(
       cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100,
-      std::max<uint64_t>(cap3 / 8, p.nmax / std::max(p.num_cus, 1u)));
+      std::max<uint64_t>(cap3 / 8, nexp / std::max(p.num_cus, 1u)));
   uint32_t b12 = 0;
-  while ((p.nmax >> b12) > target) ++b12;
+  while ((nexp >> b12) > target) ++b12;
   const uint32_t owners = p.owner_digit ? p.num_pes : 1;
   if (owners > 1024) return 0;
   uint32_t b1max = 0;
@@ -375,7 +378,7 @@ int plan_choose_path(Plan *pl, int algo) {
   // the key range.  An item over LDS capacity is redone digit by digit, and
   // its later digits' look-back slots stay unpublished meanwhile, so every
   // later item waits on it: items are sized to fit where keys are densest.
-  const double per_digit = (double)p.nmax / (double)(1ull << (b1 + b2f)) * (p.canonical ? 2.0 : 1.0);
+  const double per_digit = (double)nexp / (double)(1ull << (b1 + b2f)) * (p.canonical ? 2.0 : 1.0);
   uint32_t pack = (uint32_t)std::max(1.0, std::floor((double)target / std::max(per_digit, 1e-9)));
   pack = std::min<uint32_t>(std::min<uint32_t>(pack, 16), 1u << b2f);
   const uint32_t nb3 = fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup
@@ -396,7 +399,7 @@ int plan_choose_path(Plan *pl, int algo) {
   p.tcap1 = rt * p.S;
   // canonical k-mers (min of a k-mer and its reverse complement) are twice as
   // dense as the mean at the low end of the key range: size groups for that
-  const double per_tile_bucket = (double)p.nmax / p.ntiles1 / p.nb1 * (p.canonical ? 2.0 : 1.0);
+  const double per_tile_bucket = (double)nexp / p.ntiles1 / p.nb1 * (p.canonical ? 2.0 : 1.0);
   uint32_t G = (uint32_t)std::max(1.0, std::floor(0.75 * (double)cap2 / std::max(per_tile_bucket, 1e-9)));
   G = std::min<uint32_t>(G, L2_NT);
   G = std::min<uint32_t>(G, p.ntiles1);

@@ -373,26 +373,40 @@ int plan_choose_path(Plan *pl, int algo) {
   rt = std::max<uint32_t>(rt, 1);
   if ((uint64_t)rt * p.S > cap1) return 0;
   const uint32_t b2f = std::min(std::min(10u, bits - b1), b2 + 2);
-  // keys per digit where they are densest: canonical k-mers (the smaller of a
-  // k-mer and its reverse complement) are twice the mean at the low end of
-  // the key range.  An item over LDS capacity is redone digit by digit, and
-  // its later digits' look-back slots stay unpublished meanwhile, so every
-  // later item waits on it: items are sized to fit where keys are densest.
-  const double per_digit = (double)nexp / (double)(1ull << (b1 + b2f)) * (p.canonical ? 2.0 : 1.0);
-  uint32_t pack = (uint32_t)std::max(1.0, std::floor((double)target / std::max(per_digit, 1e-9)));
-  pack = std::min<uint32_t>(std::min<uint32_t>(pack, 16), 1u << b2f);
+  // Final items are sized to fit where keys are densest: an item over LDS
+  // capacity is redone digit by digit, and its later digits' look-back slots
+  // stay unpublished meanwhile, so every later item waits on it.  Canonical
+  // k-mers (the smaller of a k-mer and its reverse complement) fall linearly
+  // from twice the mean at the bottom of the key range to none at the top, so
+  // each quarter of an owner's top-bit range gets items sized for its own
+  // densest point (2, 1.5, 1, 0.5 x the mean).
+  const double per_digit = (double)nexp / (double)(1ull << (b1 + b2f));
+  const uint32_t lc = std::min<uint32_t>(2, b1), nd2 = 1u << b2f;
+  uint32_t packmax = 1;
+  for (uint32_t c = 0; c < 4; ++c) {
+    p.packc[c] = p.ipbc[c] = 0;
+    if (c >= (1u << lc)) continue;
+    const double dens = p.canonical ? 2.0 * (1.0 - (double)c / (double)(1u << lc)) : 1.0;
+    uint32_t pk = (uint32_t)std::max(1.0, std::floor((double)target / std::max(per_digit * dens, 1e-9)));
+    pk = std::min<uint32_t>(std::min<uint32_t>(pk, 16), nd2);
+    p.packc[c] = pk;
+    p.ipbc[c] = (nd2 + pk - 1) / pk;
+    packmax = std::max(packmax, pk);
+  }
   const uint32_t nb3 = fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup
   uint32_t b3 = 0;
-  while (b3 < 10 && b3 < bits - b1 - b2f && (pack << (b3 + 1)) <= nb3) ++b3;
+  while (b3 < 10 && b3 < bits - b1 - b2f && (packmax << (b3 + 1)) <= nb3) ++b3;
   p.b1 = b1;
   p.b2 = b2f;
   p.b3 = b3;
-  p.pack = pack;
+  p.pack = packmax;
+  p.cls_log = lc;
   p.fin_nt = fin_nt;
   p.nb1 = owners << b1;
-  p.nd2 = 1u << b2f;
-  p.ipb = (p.nd2 + pack - 1) / pack;
-  p.nsb = p.nb1 * p.ipb;
+  p.nd2 = nd2;
+  uint32_t per_block = 0;  // items of an owner block (2^b1 buckets)
+  for (uint32_t c = 0; c < 4; ++c) per_block += (1u << (b1 - lc)) * p.ipbc[c];
+  p.nsb = owners * per_block;
   p.RT1 = rt;
   p.wpr1 = wpr;
   p.ntiles1 = (uint32_t)((p.nrec + rt - 1) / rt);
@@ -554,8 +568,14 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     m.ngroups = p.ngroups;
     m.cap2 = p.cap2;
     m.fin_nt = p.fin_nt;
-    m.pack = p.pack;
-    m.ipb = p.ipb;
+    m.cls_log = p.cls_log;
+    m.bpc = 1u << (p.b1 - p.cls_log);
+    m.cpre[0] = 0;
+    for (int c = 0; c < 4; ++c) {
+      m.packc[c] = p.packc[c];
+      m.ipbc[c] = p.ipbc[c];
+      m.cpre[c + 1] = m.cpre[c] + m.bpc * p.ipbc[c];
+    }
     m.off1 = (uint16_t *)(ar + b.off_off1[j]);
     m.off2 = (uint16_t *)(ar + b.off_off2[j]);
     D.out1 = ar + b.off_out1[j];

@@ -63,7 +63,8 @@ struct Plan {
   // final item is `pack` consecutive second-level digits of a bucket (ipb
   // items per bucket, nsb items in all).
   int msd = 0;
-  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1, pack = 1, ipb = 1;
+  uint32_t b1 = 0, b2 = 0, b3 = 0, nb1 = 1, nd2 = 1, nsb = 1, pack = 1;
+  uint32_t cls_log = 0, packc[4] = {1, 0, 0, 0}, ipbc[4] = {1, 0, 0, 0};  // final-item classes (MsdArgs)
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
   // tunables set before plan_choose_path (autotuner; defaults otherwise),
   // and the CUs of the device the job runs on

@@ -121,11 +121,42 @@ struct MsdArgs {
   uint32_t ngroups;  // L2 groups per bucket
   uint32_t cap2;     // keys capacity of one (bucket, group)
   uint32_t fin_nt;   // threads of a final-level workgroup (256, 512 or 1024)
-  uint32_t pack;     // second-level digits per final item
-  uint32_t ipb;      // final items per bucket: ceil(nd2 / pack)
+  // Final items, in bucket order.  An owner block's 2^b1 buckets form
+  // 2^cls_log classes of consecutive buckets (ranges of the top bits); a
+  // class's items take packc[c] second-level digits, ipbc[c] items per bucket.
+  uint32_t cls_log;
+  uint32_t bpc;        // buckets per class: 2^(b1 - cls_log)
+  uint32_t packc[4];
+  uint32_t ipbc[4];    // ceil(nd2 / packc[c]); 0 for an unused class
+  uint32_t cpre[5];    // items of an owner block before class c; cpre[4]: per block
   uint16_t *off1;    // [nb1 + 1][ntiles1]
   uint16_t *off2;    // [nb1][ngroups][nd2 + 1]
 };
+
+// A final item: bucket b, second-level digits [s0, s0 + cnt) of it.  Its
+// keys go to LDS bin (digit - s0) << b3 | the next b3 bits, which orders
+// them as the keys order.
+struct FinItem {
+  uint32_t b, s0, cnt, nbins;
+};
+__host__ __device__ inline uint32_t sel4(const uint32_t (&v)[4], uint32_t c) {
+  return c == 0 ? v[0] : c == 1 ? v[1] : c == 2 ? v[2] : v[3];  // no indexed access to an argument
+}
+__host__ __device__ inline FinItem fin_item_of(const MsdArgs &m, uint32_t it) {
+  const uint32_t blk = it / m.cpre[4];
+  uint32_t r = it - blk * m.cpre[4];
+  // classes in order; an unused class starts at the block's end
+  const uint32_t c = (r >= m.cpre[1] ? 1u : 0u) + (r >= m.cpre[2] ? 1u : 0u) + (r >= m.cpre[3] ? 1u : 0u);
+  r -= c == 0 ? m.cpre[0] : c == 1 ? m.cpre[1] : c == 2 ? m.cpre[2] : m.cpre[3];
+  const uint32_t ipb = sel4(m.ipbc, c), pack = sel4(m.packc, c);
+  const uint32_t bi = r / ipb;
+  FinItem f;
+  f.b = ((blk << m.cls_log) + c) * m.bpc + bi;
+  f.s0 = (r - bi * ipb) * pack;
+  f.cnt = m.nd2 - f.s0 < pack ? m.nd2 - f.s0 : pack;
+  f.nbins = f.cnt << m.d3.nbits;
+  return f;
+}
 
 // One job of a multi-job MSD launch.  Work items of every stage are numbered
 // across the jobs of a launch (job j owns items [pre[j], pre[j+1])).

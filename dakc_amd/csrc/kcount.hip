@@ -1208,20 +1208,6 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
 // Publish sub-bucket sb's distinct count H and return its exclusive prefix
 // over the earlier sub-buckets (decoupled look-back by wavefront 0).  Called
 // by the whole workgroup; ends with a barrier.
-// A final item: bucket b, second-level digits [s0, s0 + cnt) of it.  Its
-// keys go to LDS bin (digit - s0) << b3 | the next b3 bits, which orders
-// them as the keys order.
-struct FinItem {
-  uint32_t b, s0, cnt, nbins;
-};
-DK_DEV FinItem fin_item_of(const MsdArgs &m, uint32_t it) {
-  FinItem f;
-  f.b = it / m.ipb;
-  f.s0 = (it - f.b * m.ipb) * m.pack;
-  f.cnt = m.nd2 - f.s0 < m.pack ? m.nd2 - f.s0 : m.pack;
-  f.nbins = f.cnt << m.d3.nbits;
-  return f;
-}
 template <typename KeyT>
 DK_DEV uint32_t fin_bin(const MsdArgs &m, const FinItem &f, const KeyT &x) {
   return ((key_bits(x, m.d2.shift, m.d2.nbits) - f.s0) << m.d3.nbits) | key_bits(x, m.d3.shift, m.d3.nbits);

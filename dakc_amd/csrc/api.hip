@@ -568,13 +568,12 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
     m.ngroups = p.ngroups;
     m.cap2 = p.cap2;
     m.fin_nt = p.fin_nt;
-    m.cls_log = p.cls_log;
-    m.bpc = 1u << (p.b1 - p.cls_log);
-    m.cpre[0] = 0;
+    m.cls = p.cls_log | (p.b1 - p.cls_log) << 4;
+    m.packc = 0;
+    m.ipbc[0] = m.ipbc[1] = 0;
     for (int c = 0; c < 4; ++c) {
-      m.packc[c] = p.packc[c];
-      m.ipbc[c] = p.ipbc[c];
-      m.cpre[c + 1] = m.cpre[c] + m.bpc * p.ipbc[c];
+      m.packc |= p.packc[c] << (8 * c);
+      m.ipbc[c >> 1] |= p.ipbc[c] << (16 * (c & 1));
     }
     m.off1 = (uint16_t *)(ar + b.off_off1[j]);
     m.off2 = (uint16_t *)(ar + b.off_off2[j]);

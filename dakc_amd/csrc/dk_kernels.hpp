@@ -121,14 +121,13 @@ struct MsdArgs {
   uint32_t ngroups;  // L2 groups per bucket
   uint32_t cap2;     // keys capacity of one (bucket, group)
   uint32_t fin_nt;   // threads of a final-level workgroup (256, 512 or 1024)
-  // Final items, in bucket order.  An owner block's 2^b1 buckets form
-  // 2^cls_log classes of consecutive buckets (ranges of the top bits); a
-  // class's items take packc[c] second-level digits, ipbc[c] items per bucket.
-  uint32_t cls_log;
-  uint32_t bpc;        // buckets per class: 2^(b1 - cls_log)
-  uint32_t packc[4];
-  uint32_t ipbc[4];    // ceil(nd2 / packc[c]); 0 for an unused class
-  uint32_t cpre[5];    // items of an owner block before class c; cpre[4]: per block
+  // Final items, in bucket order.  An owner block's 2^b1 buckets form 2^cl
+  // classes of 2^bl consecutive buckets each (ranges of the top bits); class
+  // c's items take pack_c second-level digits, ipb_c items per bucket.  Kept
+  // in four words (few scalar registers in the final level's loop):
+  uint32_t cls;    // cl | bl << 4
+  uint32_t packc;  // pack_c in byte c
+  uint32_t ipbc[2];  // ipb_c = ceil(nd2 / pack_c) in halfword c (0: class unused)
   uint16_t *off1;    // [nb1 + 1][ntiles1]
   uint16_t *off2;    // [nb1][ngroups][nd2 + 1]
 };
@@ -139,19 +138,21 @@ struct MsdArgs {
 struct FinItem {
   uint32_t b, s0, cnt, nbins;
 };
-__host__ __device__ inline uint32_t sel4(const uint32_t (&v)[4], uint32_t c) {
-  return c == 0 ? v[0] : c == 1 ? v[1] : c == 2 ? v[2] : v[3];  // no indexed access to an argument
-}
 __host__ __device__ inline FinItem fin_item_of(const MsdArgs &m, uint32_t it) {
-  const uint32_t blk = it / m.cpre[4];
-  uint32_t r = it - blk * m.cpre[4];
-  // classes in order; an unused class starts at the block's end
-  const uint32_t c = (r >= m.cpre[1] ? 1u : 0u) + (r >= m.cpre[2] ? 1u : 0u) + (r >= m.cpre[3] ? 1u : 0u);
-  r -= c == 0 ? m.cpre[0] : c == 1 ? m.cpre[1] : c == 2 ? m.cpre[2] : m.cpre[3];
-  const uint32_t ipb = sel4(m.ipbc, c), pack = sel4(m.packc, c);
+  const uint32_t cl = m.cls & 15, bl = (m.cls >> 4) & 15;
+  const uint32_t i0 = m.ipbc[0] & 0xFFFFu, i1 = m.ipbc[0] >> 16, i2 = m.ipbc[1] & 0xFFFFu, i3 = m.ipbc[1] >> 16;
+  // items of an owner block before classes 1, 2, 3 and in all of it
+  const uint32_t e0 = i0 << bl, e1 = e0 + (i1 << bl), e2 = e1 + (i2 << bl), e3 = e2 + (i3 << bl);
+  const uint32_t blk = it / e3;
+  uint32_t r = it - blk * e3;
+  // classes in order; an unused class has no items and is never chosen
+  const uint32_t c = (r >= e0 ? 1u : 0u) + (r >= e1 ? 1u : 0u) + (r >= e2 ? 1u : 0u);
+  r -= c == 0 ? 0u : c == 1 ? e0 : c == 2 ? e1 : e2;
+  const uint32_t ipb = c == 0 ? i0 : c == 1 ? i1 : c == 2 ? i2 : i3;
+  const uint32_t pack = (m.packc >> (8 * c)) & 0xFFu;
   const uint32_t bi = r / ipb;
   FinItem f;
-  f.b = ((blk << m.cls_log) + c) * m.bpc + bi;
+  f.b = (((blk << cl) + c) << bl) + bi;
   f.s0 = (r - bi * ipb) * pack;
   f.cnt = m.nd2 - f.s0 < pack ? m.nd2 - f.s0 : pack;
   f.nbins = f.cnt << m.d3.nbits;

@@ -15,9 +15,9 @@ int main() {
             if ((owners << b1) > 1024) continue;
             MsdArgs m{};
             const uint32_t lc = b1 < 2 ? b1 : 2, nd2 = 1u << b2;
-            m.nb1 = owners << b1; m.nd2 = nd2; m.cls_log = lc; m.bpc = 1u << (b1 - lc);
+            m.nb1 = owners << b1; m.nd2 = nd2; m.cls = lc | (b1 - lc) << 4;
             m.d3.nbits = 3;
-            m.cpre[0] = 0;
+            uint32_t per_block = 0;
             for (uint32_t c = 0; c < 4; ++c) {
               uint32_t pk = 0, ip = 0;
               if (c < (1u << lc)) {
@@ -26,10 +26,11 @@ int main() {
                 if (pk > nd2) pk = nd2;
                 ip = (nd2 + pk - 1) / pk;
               }
-              m.packc[c] = pk; m.ipbc[c] = ip;
-              m.cpre[c + 1] = m.cpre[c] + m.bpc * ip;
+              m.packc |= pk << (8 * c);
+              m.ipbc[c >> 1] |= ip << (16 * (c & 1));
+              per_block += ip << (b1 - lc);
             }
-            const uint32_t n = owners * m.cpre[4];
+            const uint32_t n = owners * per_block;
             uint64_t expect = 0;  // next (b * nd2 + digit) to cover
             for (uint32_t it = 0; it < n; ++it) {
               const FinItem f = fin_item_of(m, it);

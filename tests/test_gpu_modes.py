@@ -31,6 +31,22 @@ def test_batch_matches_plain_and_oracle():
             assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
 
 
+def test_batch_and_serve_canonical_match_oracle():
+    """Canonical jobs, whose final items are sized per quarter of the key
+    range, in one batch with a low solo threshold (some jobs get final
+    launches of their own, the rest share the multi-job launch), then served
+    side by side: every table equals the oracle's."""
+    sets = read_sets(5, 8)
+    want = [oracle.count(s, 250, 31, True) for s in sets]
+    with KmerCounter(k=31, read_len=250, canonical=True, batch_solo_items=64, max_inflight=3) as kc:
+        for t, (keys, counts, n, _) in zip(kc.count_batch(sets), want):
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+        tickets = [kc.submit(s) for s in sets]
+        for (tk, _buf), (keys, counts, n, _) in zip(tickets, want):
+            t = kc.wait(tk)
+            assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+
+
 def test_served_mix_equals_alone():
     sets = read_sets(2, 10)
     ks = [21, 31, 55] * 4

@@ -49,7 +49,7 @@ class Job(C.Structure):
         ("reads", C.c_void_p), ("nbytes", C.c_uint64), ("format", C.c_int32),
         ("read_len", C.c_int32), ("k", C.c_int32), ("canonical", C.c_int32),
         ("num_pes", C.c_uint32), ("owner_bin", C.c_int32), ("reads_on_device", C.c_int32),
-        ("table_on_device", C.c_int32),
+        ("table_on_device", C.c_int32), ("hash_seed", C.c_uint64),
     ]
 
 
@@ -58,6 +58,7 @@ class Table(C.Structure):
         ("keys", C.POINTER(C.c_uint64)), ("counts", C.POINTER(C.c_uint32)), ("n", C.c_uint64),
         ("total", C.c_uint64), ("bin_offsets", C.POINTER(C.c_uint64)), ("num_bins", C.c_uint32),
         ("key_words", C.c_int32), ("k", C.c_int32), ("on_device", C.c_int32), ("priv", C.c_void_p),
+        ("hash_seed", C.c_uint64),
     ]
 
 

@@ -5,7 +5,7 @@
 // the reference fixes at compile time (KMERLEN, READLEN, dakc/Makefile:11) are
 // flags here.  Unlike the reference, the table is written out (-o).
 //
-//   dakc -f reads.txt [-k 31] [-r 150] [-c] [-p P] [-b bin] [-o table.bin]
+//   dakc -f reads.txt [-k 31] [-r 150] [-c] [-p P] [-b bin] [-s seed] [-o table.bin]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,13 +17,14 @@
 
 static void usage() {
   std::printf(
-      "usage: dakc -f <reads.txt> [-k K] [-r READLEN] [-c] [-p P] [-b BIN] [-o out.bin] [-d DEV]\n"
+      "usage: dakc -f <reads.txt> [-k K] [-r READLEN] [-c] [-p P] [-b BIN] [-s SEED] [-o out.bin] [-d DEV]\n"
       "  -f  record file: READLEN bytes + '\\n' per read ('M'-padded, as syngen/fq2txtmaker.sh)\n"
       "  -k  k-mer length 1..64 (default 31)\n"
       "  -r  record width READLEN (default 150)\n"
       "  -c  canonical k-mers (min of k-mer and reverse complement)\n"
       "  -p  number of PEs P for the owner hash (default 1)\n"
       "  -b  count only the k-mers owned by PE BIN (default: all bins)\n"
+      "  -s  owner-hash seed (default 0: the reference's 0x9E3779B97F4A7C15)\n"
       "  -o  write the table: u64 kmer (or u64 lo,u64 hi) + u32 count per entry\n");
 }
 
@@ -31,8 +32,9 @@ int main(int argc, char **argv) {
   std::string file, out;
   int k = 31, read_len = 150, canonical = 0, dev = 0;
   long p = 1, bin = -1;
+  unsigned long long seed = 0;
   int opt;
-  while ((opt = getopt(argc, argv, "hf:k:r:cp:b:o:d:")) != -1) {
+  while ((opt = getopt(argc, argv, "hf:k:r:cp:b:s:o:d:")) != -1) {
     switch (opt) {
     case 'f': file = optarg; break;
     case 'k': k = std::atoi(optarg); break;
@@ -40,6 +42,7 @@ int main(int argc, char **argv) {
     case 'c': canonical = 1; break;
     case 'p': p = std::atol(optarg); break;
     case 'b': bin = std::atol(optarg); break;
+    case 's': seed = std::strtoull(optarg, nullptr, 0); break;
     case 'o': out = optarg; break;
     case 'd': dev = std::atoi(optarg); break;
     case 'h':
@@ -58,7 +61,7 @@ int main(int argc, char **argv) {
     std::vector<char> chunk = dakc::fqreader(file, read_len).read_file();
     auto t1 = std::chrono::steady_clock::now();
     std::printf("Reading time: %.6f seconds\n", std::chrono::duration<double>(t1 - t0).count());
-    dakc::kmercounter km(ctx, chunk, k, read_len, canonical != 0, (uint32_t)p, (int32_t)bin);
+    dakc::kmercounter km(ctx, chunk, k, read_len, canonical != 0, (uint32_t)p, (int32_t)bin, seed);
     auto t2 = std::chrono::steady_clock::now();
     std::printf("kmer counting time: %.6f seconds\n", std::chrono::duration<double>(t2 - t1).count());
     std::printf("global_kmers: %llu\nglobal_distinct_kmers: %llu\n", (unsigned long long)km.total(),

@@ -294,6 +294,7 @@ int make_plan(const dakc_job *job, Plan *pl) {
   p.nbytes = job->nbytes;
   p.num_pes = job->num_pes;
   p.owner_bin = job->num_pes > 1 ? job->owner_bin : -1;
+  p.seed = job->hash_seed ? job->hash_seed : OWNER_SEED;
   const uint64_t rem = p.nbytes % p.stride;
   p.nrec = p.nbytes / p.stride;
   p.last_has_nl = 1;
@@ -441,6 +442,7 @@ ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads) {
   rd.num_pes = p.num_pes;
   rd.owner_bin = p.owner_bin;
   rd.last_has_nl = p.last_has_nl;
+  rd.seed = p.seed;
   return rd;
 }
 
@@ -660,8 +662,8 @@ void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
   }
   for (size_t j = 0; j < nj; ++j)
     if (b.plans[j].owner_digit)
-      launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.locs[j].binoff,
-                            st);
+      launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.plans[j].seed,
+                            b.locs[j].binoff, st);
   if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
 }
 
@@ -733,7 +735,7 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
     }
     const uint32_t ep = ws.next_epoch();
     launch_rle(pl.kw, cur, other, w, ep, pl.tiles_keys, st);
-    if (pl.owner_digit) launch_bin_offsets(pl.kw, other, pl.num_pes, w, st);
+    if (pl.owner_digit) launch_bin_offsets(pl.kw, other, pl.num_pes, pl.seed, w, st);
   }
   if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
   ws.loc.keys = other;
@@ -914,6 +916,7 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   out->key_words = pl.kw;
   out->k = pl.k;
   out->num_bins = pl.owner_digit ? pl.num_pes : 0;
+  out->hash_seed = pl.seed;
   if (pl.owner_digit) {
     out->bin_offsets = (uint64_t *)std::malloc(((size_t)pl.num_pes + 1) * 8);
     if (!out->bin_offsets) return DAKC_E_NOMEM;

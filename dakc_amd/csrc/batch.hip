@@ -57,6 +57,7 @@ static int table_to_host(const MsdBatch &b, size_t j, uint64_t n, uint64_t d, da
   out->key_words = p.kw;
   out->k = p.k;
   out->num_bins = p.owner_digit ? p.num_pes : 0;
+  out->hash_seed = p.seed;
   const size_t kb = (size_t)d * p.kw * 8;
   out->keys = (uint64_t *)std::malloc(kb ? kb : 8);
   out->counts = (uint32_t *)std::malloc(d ? d * 4 : 4);

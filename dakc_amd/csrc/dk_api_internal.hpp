@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/dakc.h"
+#include "dk_device.hpp"
 #include "dk_kernels.hpp"
 
 namespace dk {
@@ -56,6 +57,7 @@ struct Plan {
   uint32_t tiles0 = 0, tiles_keys = 0;
   uint32_t num_pes = 1;
   int32_t owner_bin = -1;
+  uint64_t seed = OWNER_SEED;  // owner-hash seed
   int ndig = 0, owner_digit = 0, npasses = 0;
   uint32_t canonical = 0, last_has_nl = 1;
   // MSD path (msd = 1): first-level bits b1 (with the owner when all bins

@@ -61,11 +61,12 @@ DK_DEV uint64_t murmur64a(const K2 &key, uint64_t seed) {
   h ^= h >> 47;
   return h;
 }
-constexpr uint64_t OWNER_SEED = 0x9E3779B97F4A7C15ULL;  // kcounter.cpp:52
+constexpr uint64_t OWNER_SEED = 0x9E3779B97F4A7C15ULL;  // kcounter.cpp:52, the default seed
 
+// owner PE of a k-mer (kcounter.cpp:50-54 with the job's seed)
 template <typename KeyT>
-DK_DEV uint32_t owner_of(const KeyT &x, uint32_t num_pes) {
-  return (uint32_t)(murmur64a(x, OWNER_SEED) % (uint64_t)num_pes);
+DK_DEV uint32_t owner_of(const KeyT &x, uint32_t num_pes, uint64_t seed) {
+  return (uint32_t)(murmur64a(x, seed) % (uint64_t)num_pes);
 }
 
 // bits [shift, shift + nbits) of the key value, nbits <= 32

@@ -23,6 +23,7 @@ struct ReadsDesc {
   uint32_t num_pes;    // P
   int32_t owner_bin;   // -1: all bins; >= 0: keep only k-mers owned by this bin
   uint32_t last_has_nl;  // 0 when the final record has no trailing '\n'
+  uint64_t seed;         // owner-hash seed
 };
 
 // error bits reported through the job's err word
@@ -190,9 +191,9 @@ void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc
                       uint32_t n2, uint32_t n3, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
-void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,
+void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, uint64_t seed, const Workspace &ws,
                         hipStream_t st);
-void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes,
+void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes, uint64_t seed,
                            uint64_t *offsets, hipStream_t st);
 
 }  // namespace dk

@@ -222,7 +222,7 @@ DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys
       if (v) {
         key = window_key<KeyT>(sm.code[base], sm.code[base + 1], sm.code[base + 2], 2 * sh, k);
         if (rd.canonical) key = canon(key, k);
-        if (rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes) != (uint32_t)rd.owner_bin)
+        if (rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes, rd.seed) != (uint32_t)rd.owner_bin)
           v = false;
       }
     }
@@ -234,10 +234,10 @@ DK_DEV void extract_tile(const ReadsDesc &rd, uint32_t tile, SM &sm, KeyT (&keys
 // ------------------------------------------------------------- histogram --
 
 template <typename KeyT>
-DK_DEV uint32_t digit_eval(const DigitSpec &ds, const KeyT &key) {
-  if (ds.mode == DIG_OWNER) return owner_of(key, ds.num_pes);
+DK_DEV uint32_t digit_eval(const DigitSpec &ds, const KeyT &key, uint64_t seed) {
+  if (ds.mode == DIG_OWNER) return owner_of(key, ds.num_pes, seed);
   uint32_t v = key_bits(key, ds.shift, ds.nbits);
-  if (ds.mode == DIG_MSD && ds.num_pes > 1) v += owner_of(key, ds.num_pes) << ds.nbits;
+  if (ds.mode == DIG_MSD && ds.num_pes > 1) v += owner_of(key, ds.num_pes, seed) << ds.nbits;
   return v;
 }
 
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_hist_reads(ReadsDesc rd, HistSpecs hs, u
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if (!((vmask >> j) & 1)) continue;
-      for (int p = 0; p < hs.n; ++p) atomicAdd(&s_h[p * 256 + digit_eval(hs.d[p], keys[j])], 1u);
+      for (int p = 0; p < hs.n; ++p) atomicAdd(&s_h[p * 256 + digit_eval(hs.d[p], keys[j], rd.seed)], 1u);
     }
     __syncthreads();
   }
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
   for (int j = 0; j < ITEMS; ++j) {
     const bool v = (vmask >> j) & 1;
     uint32_t d = 0;
-    if (v) d = digit_eval(a.dig, keys[j]);
+    if (v) d = digit_eval(a.dig, keys[j], a.rd.seed);
     uint64_t peers = __ballot(v);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -526,14 +526,14 @@ __global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint
 // owner-bin boundaries of a table sorted by (owner, kmer): lower bound search
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uint64_t *n_dev,
-                                                     uint32_t num_pes, uint64_t *offsets) {
+                                                     uint32_t num_pes, uint64_t seed, uint64_t *offsets) {
   const uint64_t d = n_dev[1];
   for (uint32_t o = threadIdx.x; o <= num_pes; o += blockDim.x) {
     uint64_t lo = 0, hi = d;
     if (o == num_pes) lo = d;
     while (lo < hi) {
       const uint64_t mid = lo + ((hi - lo) >> 1);
-      if (owner_of(keys[mid], num_pes) < o) lo = mid + 1;
+      if (owner_of(keys[mid], num_pes, seed) < o) lo = mid + 1;
       else hi = mid;
     }
     offsets[o] = lo;
@@ -971,7 +971,7 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
         (rd.L < 2 || ((sm.inv[rb + ((p - 1) >> 5)] >> ((p - 1) & 31)) & 1)))
       v = false;
     KeyT key = rd.canonical ? (key_lt(rc, fw) ? rc : fw) : fw;
-    if (v && rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes) != (uint32_t)rd.owner_bin)
+    if (v && rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes, rd.seed) != (uint32_t)rd.owner_bin)
       v = false;
     keys[j] = key;
     vmask |= (v ? 1u : 0u) << j;
@@ -1050,7 +1050,7 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         if ((vmask >> j) & 1) {
-          const uint32_t d = digit_eval(m.d1, keys[j]);
+          const uint32_t d = digit_eval(m.d1, keys[j], rd.seed);
           dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
         }
       }
@@ -1692,20 +1692,16 @@ void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws,
 }
 
 void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes,
-                           uint64_t *offsets, hipStream_t st) {
+                           uint64_t seed, uint64_t *offsets, hipStream_t st) {
   if (kw == 1)
-    k_bin_offsets<uint64_t><<<1, 256, 0, st>>>((const uint64_t *)table_keys, nd, num_pes, offsets);
+    k_bin_offsets<uint64_t><<<1, 256, 0, st>>>((const uint64_t *)table_keys, nd, num_pes, seed, offsets);
   else
-    k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, nd, num_pes, offsets);
+    k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, nd, num_pes, seed, offsets);
 }
 
-void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, const Workspace &ws,
+void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, uint64_t seed, const Workspace &ws,
                         hipStream_t st) {
-  if (kw == 1)
-    k_bin_offsets<uint64_t><<<1, 256, 0, st>>>((const uint64_t *)table_keys, ws.n_dev, num_pes,
-                                               ws.bin_offsets);
-  else
-    k_bin_offsets<K2><<<1, 256, 0, st>>>((const K2 *)table_keys, ws.n_dev, num_pes, ws.bin_offsets);
+  launch_bin_offsets_at(kw, table_keys, ws.n_dev, num_pes, seed, ws.bin_offsets, st);
 }
 
 // compute units of the stream's device (one persistent L1 workgroup each)

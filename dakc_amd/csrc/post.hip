@@ -190,6 +190,7 @@ struct QueryArgs {
   uint64_t d;
   uint32_t nbins;           // 0: one sorted run
   const uint64_t *offs;     // nbins + 1
+  uint64_t seed;            // owner-hash seed of the bins
   const void *queries;
   uint64_t nq;
   uint32_t *out;
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a) {
   const KeyT q = reinterpret_cast<const KeyT *>(a.queries)[i];
   uint64_t lo = 0, end = a.d;
   if (a.nbins) {
-    const uint32_t o = owner_of(q, a.nbins);
+    const uint32_t o = owner_of(q, a.nbins, a.seed);
     lo = a.offs[o];
     end = a.offs[o + 1];
   }
@@ -332,6 +333,7 @@ extern "C" int dakc_table_query(dakc_ctx *ctx, const dakc_table *t, const uint64
   a.d = t->n;
   a.nbins = nbins;
   a.offs = (const uint64_t *)p;
+  a.seed = t->hash_seed ? t->hash_seed : OWNER_SEED;
   a.nq = nq;
   a.queries = on_device ? (const void *)queries : (const void *)(p + qoff);
   a.out = on_device ? counts_out : (uint32_t *)(p + coff);

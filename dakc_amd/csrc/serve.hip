@@ -186,6 +186,7 @@ static void retire(ServeState &s, Slot &sl) {
   o.key_words = t.plan.kw;
   o.k = t.plan.k;
   o.num_bins = t.plan.owner_digit ? t.plan.num_pes : 0;
+  o.hash_seed = t.plan.seed;
   o.on_device = 1;
   h->arena = s.arena;
   h->resident = s.resident;

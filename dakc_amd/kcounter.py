@@ -78,11 +78,11 @@ class KmerCounter:
     """One GPU context; count() replaces kmercounter(read_chunk, vectordbg)."""
 
     def __init__(self, k: int = 31, read_len: int = 150, *, canonical: bool = False,
-                 num_pes: int = 1, owner_bin: int = -1, device: int = 0,
+                 num_pes: int = 1, owner_bin: int = -1, hash_seed: int = 0, device: int = 0,
                  hbm_budget: int = 0, max_inflight: int = 0, algo: int = 0, max_large: int = 0,
                  batch_solo_items: int = 0):
         self.k, self.read_len, self.canonical = k, read_len, canonical
-        self.num_pes, self.owner_bin = num_pes, owner_bin
+        self.num_pes, self.owner_bin, self.hash_seed = num_pes, owner_bin, hash_seed
         self._libh = _lib.load()
         self._ctx = C.c_void_p()
         opts = Opts(hbm_budget=hbm_budget, max_inflight=max_inflight, algo=algo, max_large=max_large,
@@ -114,12 +114,12 @@ class KmerCounter:
     def job(self, reads_ptr: int, nbytes: int, *, on_device: bool = False, **over) -> Job:
         p = dict(k=self.k, read_len=self.read_len, canonical=self.canonical,
                  num_pes=self.num_pes, owner_bin=self.owner_bin, table_on_device=False,
-                 format=_lib.FMT_FIXED)
+                 format=_lib.FMT_FIXED, hash_seed=self.hash_seed)
         p.update(over)
         return Job(reads=reads_ptr, nbytes=nbytes, format=p["format"], read_len=p["read_len"],
                    k=p["k"], canonical=1 if p["canonical"] else 0, num_pes=p["num_pes"],
                    owner_bin=p["owner_bin"], reads_on_device=1 if on_device else 0,
-                   table_on_device=1 if p["table_on_device"] else 0)
+                   table_on_device=1 if p["table_on_device"] else 0, hash_seed=p["hash_seed"])
 
     def count(self, reads, *, dev_ptr: int = 0, nbytes: int = 0, **over) -> KmerTable:
         """Count one read buffer into a host table.  With dev_ptr the reads are

@@ -19,8 +19,9 @@
  *   - k-mers: forward strand, first base in the highest bits, 2k bits
  *     (kcounter_funcs.cpp:25-67); canonical = min(kmer, reverse complement)
  *     on request (no reference counterpart);
- *   - owner: MurmurHash64A(kmer, 0x9E3779B97F4A7C15) % num_pes
- *     (kcounter.cpp:29-54); for 2x64-bit keys MurmurHash64A over lo||hi;
+ *   - owner: MurmurHash64A(kmer, seed) % num_pes with the reference's seed
+ *     0x9E3779B97F4A7C15 unless the job names another (kcounter.cpp:29-54);
+ *     for 2x64-bit keys MurmurHash64A over lo||hi;
  *   - output: ascending by k-mer (within each owner bin), no duplicates,
  *     counts >= 1 (kcounter.cpp:385-411).
  */
@@ -34,7 +35,7 @@
 extern "C" {
 #endif
 
-#define DAKC_ABI_VERSION 1
+#define DAKC_ABI_VERSION 2
 
 typedef enum dakc_status {
   DAKC_OK = 0,
@@ -71,6 +72,8 @@ typedef struct dakc_job {
   int32_t table_on_device; /* served mode: dakc_wait returns the table in device
                             * memory owned by the table (keys/counts are device
                             * pointers; dakc_table_free releases it) */
+  uint64_t hash_seed;  /* seed of the owner hash; 0 = the reference's
+                        * 0x9E3779B97F4A7C15 (kcounter.cpp:52) */
 } dakc_job;
 
 /* The (kmer, count) table.  keys holds n * key_words u64 (lo, hi per key for
@@ -92,6 +95,7 @@ typedef struct dakc_table {
   int32_t k;
   int32_t on_device;    /* 1: keys/counts are device pointers (dakc_count_device) */
   void *priv;
+  uint64_t hash_seed;   /* owner-hash seed the bins were formed with (queries use it) */
 } dakc_table;
 
 typedef struct dakc_ctx dakc_ctx;

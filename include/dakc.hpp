@@ -72,7 +72,8 @@ class fqreader {
 class kmercounter {
  public:
   kmercounter(const context &ctx, const std::vector<char> &read_chunk, int k, int read_len,
-              bool canonical = false, uint32_t num_pes = 1, int32_t owner_bin = -1) {
+              bool canonical = false, uint32_t num_pes = 1, int32_t owner_bin = -1,
+              uint64_t hash_seed = 0) {
     dakc_job job{};
     job.reads = read_chunk.data();
     job.nbytes = read_chunk.size();
@@ -82,6 +83,7 @@ class kmercounter {
     job.canonical = canonical ? 1 : 0;
     job.num_pes = num_pes;
     job.owner_bin = owner_bin;
+    job.hash_seed = hash_seed;  // 0: the reference's owner seed (kcounter.cpp:52)
     check(dakc_count(ctx.get(), &job, &table_), "dakc_count");
   }
   ~kmercounter() { dakc_table_free(&table_); }

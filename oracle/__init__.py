@@ -40,14 +40,21 @@ def load():
     lib.dko_murmur64a_u128.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
     lib.dko_extract.restype = C.c_int64
     lib.dko_extract.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
-                                C.c_int32, C.c_void_p, C.c_uint64]
+                                C.c_int32, C.c_uint64, C.c_void_p, C.c_uint64]
     lib.dko_count.restype = C.c_int
     lib.dko_count.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
-                              C.c_int32, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64),
+                              C.c_int32, C.c_uint64, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64),
                               P(C.c_uint64), C.c_void_p]
     lib.dko_count_mt.restype = C.c_int
-    lib.dko_count_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 P(C.c_uint64), P(C.c_uint64), P(C.c_uint64), P(C.c_double)]
+    lib.dko_count_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int32,
+                                 C.c_uint64, C.c_int, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                                 P(C.c_double)]
+    lib.dko_count_keys_mt.restype = C.c_int
+    lib.dko_count_keys_mt.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, P(C.c_uint64),
+                                      P(C.c_uint64), P(C.c_double)]
+    lib.dko_extract_mt.restype = C.c_int64
+    lib.dko_extract_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                   C.c_int32, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
     lib.dko_table_checksum.restype = C.c_uint64
     lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     lib.dko_free.restype = None
@@ -62,7 +69,7 @@ def load():
                                C.c_uint32, P(C.c_uint64)]
     lib.dko_query.restype = None
     lib.dko_query.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint32,
-                              C.c_void_p, C.c_uint64, C.c_void_p]
+                              C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = lib
     return lib
 
@@ -79,21 +86,45 @@ class OracleError(RuntimeError):
         super().__init__(f"oracle status {status}")
 
 
-def extract(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1) -> np.ndarray:
+def extract(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1, seed=0) -> np.ndarray:
+    """Every k-mer of the records in record order, (n, key_words) u64."""
     lib = load()
     b = _buf(reads)
     ptr = b.ctypes.data if b.size else None
-    n = lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, None, 0)
+    n = lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed, None, 0)
     if n < 0:
         raise OracleError(-n)
     kw = 1 if k <= 32 else 2
     out = np.zeros((max(n, 1), kw), np.uint64)
-    lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin,
+    lib.dko_extract(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed,
                     out.ctypes.data, n)
     return out[:n]
 
 
-def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1):
+def extract_mt(reads, read_len, k, threads, canonical=False, num_pes=1, owner_bin=-1, seed=0,
+               cap=None) -> np.ndarray:
+    """extract() on `threads` CPU threads (same keys, same order)."""
+    lib = load()
+    b = _buf(reads)
+    ptr = b.ctypes.data if b.size else None
+    kw = 1 if k <= 32 else 2
+    if cap is None:
+        n = lib.dko_extract_mt(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed, threads,
+                               None, 0)
+        if n < 0:
+            raise OracleError(-n)
+        cap = n
+    out = np.zeros((max(cap, 1), kw), np.uint64)
+    n = lib.dko_extract_mt(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed, threads,
+                           out.ctypes.data, cap)
+    if n < 0:
+        raise OracleError(-n)
+    if n > cap:
+        raise OracleError(3)
+    return out[:n]
+
+
+def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1, seed=0):
     """Returns (keys (d, kw) u64, counts (d,) u32, total n, bin_offsets or None)."""
     lib = load()
     b = _buf(reads)
@@ -104,7 +135,7 @@ def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1):
     n = C.c_uint64()
     all_bins = num_pes > 1 and owner_bin < 0
     offs = np.zeros(num_pes + 1, np.uint64) if all_bins else None
-    st = lib.dko_count(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, C.byref(keys),
+    st = lib.dko_count(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed, C.byref(keys),
                        C.byref(counts), C.byref(d), C.byref(n),
                        offs.ctypes.data if offs is not None else None)
     if st:
@@ -124,17 +155,32 @@ def count(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1):
     return kk, cc, int(n.value), offs
 
 
-def count_mt(reads, read_len, k, threads, canonical=False):
+def count_mt(reads, read_len, k, threads, canonical=False, num_pes=1, owner_bin=-1, seed=0):
     """DAKC's P-PE shape on `threads` CPU threads; returns (n, d, checksum, seconds)."""
     lib = load()
     b = _buf(reads)
     n, d, cs, el = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
     st = lib.dko_count_mt(b.ctypes.data if b.size else None, b.size, read_len, k,
-                          1 if canonical else 0, threads,
+                          1 if canonical else 0, num_pes, owner_bin, seed, threads,
                           C.byref(n), C.byref(d), C.byref(cs), C.byref(el))
     if st:
         raise OracleError(st)
     return int(n.value), int(d.value), int(cs.value), float(el.value)
+
+
+def count_keys_mt(keys: np.ndarray, k: int, threads: int):
+    """Sort + run-length reduce of a key array on `threads` CPU threads;
+    returns (d, checksum, seconds)."""
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    kw = 1 if k <= 32 else 2
+    n = keys.size // kw
+    d, cs, el = C.c_uint64(), C.c_uint64(), C.c_double()
+    st = lib.dko_count_keys_mt(keys.ctypes.data if n else None, n, k, threads, C.byref(d), C.byref(cs),
+                               C.byref(el))
+    if st:
+        raise OracleError(st)
+    return int(d.value), int(cs.value), float(el.value)
 
 
 def checksum(keys: np.ndarray, counts: np.ndarray) -> int:
@@ -179,7 +225,7 @@ def filter_table(keys: np.ndarray, counts: np.ndarray, min_count: int, bin_offse
     return keys, counts[:n], int(total.value), offs
 
 
-def query(keys: np.ndarray, counts: np.ndarray, queries: np.ndarray, bin_offsets=None) -> np.ndarray:
+def query(keys: np.ndarray, counts: np.ndarray, queries: np.ndarray, bin_offsets=None, seed=0) -> np.ndarray:
     """Count of each query (0 when absent) in a sorted table."""
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
     kw = keys.shape[1] if keys.ndim == 2 else 1
@@ -189,7 +235,7 @@ def query(keys: np.ndarray, counts: np.ndarray, queries: np.ndarray, bin_offsets
     out = np.zeros(queries.shape[0], dtype=np.uint32)
     load().dko_query(keys.ctypes.data, counts.ctypes.data, counts.size, kw,
                      offs.ctypes.data if offs is not None else None,
-                     (offs.size - 1) if offs is not None else 0, queries.ctypes.data, queries.shape[0],
+                     (offs.size - 1) if offs is not None else 0, seed, queries.ctypes.data, queries.shape[0],
                      out.ctypes.data)
     return out
 

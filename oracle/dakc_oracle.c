@@ -126,18 +126,28 @@ typedef struct {
     int k, canonical, kw;
     uint32_t num_pes;
     int32_t owner_bin;
+    uint64_t seed;
     uint64_t *out;
     uint64_t cap, n;
 } emit_ctx;
 
-static inline uint32_t owner_of(u128 key, int kw, uint32_t num_pes) {
-    uint64_t h = kw == 1 ? dko_murmur64a_u64((uint64_t)key, DKO_OWNER_SEED)
-                         : dko_murmur64a_u128((uint64_t)key, (uint64_t)(key >> 64), DKO_OWNER_SEED);
-    return (uint32_t)(h % num_pes);
+/* the seed a job names, 0 standing for the reference's (kcounter.cpp:52) */
+static inline uint64_t seed_or_default(uint64_t seed) { return seed ? seed : DKO_OWNER_SEED; }
+
+/* MurmurHash64A of a key of kw words (kcounter.cpp:29-48 for kw = 1) */
+static inline uint64_t key_hash(u128 key, int kw, uint64_t seed) {
+    return kw == 1 ? dko_murmur64a_u64((uint64_t)key, seed)
+                   : dko_murmur64a_u128((uint64_t)key, (uint64_t)(key >> 64), seed);
+}
+
+/* kcounter.cpp:50-54 (owner_pe) */
+static inline uint32_t owner_of(u128 key, int kw, uint32_t num_pes, uint64_t seed) {
+    return (uint32_t)(key_hash(key, kw, seed) % num_pes);
 }
 
 static inline void emit_key(emit_ctx *c, u128 key) {
-    if (c->num_pes > 1 && c->owner_bin >= 0 && owner_of(key, c->kw, c->num_pes) != (uint32_t)c->owner_bin)
+    if (c->num_pes > 1 && c->owner_bin >= 0 &&
+        owner_of(key, c->kw, c->num_pes, c->seed) != (uint32_t)c->owner_bin)
         return;
     if (c->out && c->n < c->cap) {
         c->out[c->n * c->kw] = (uint64_t)key;
@@ -194,12 +204,12 @@ static int64_t extract_range(const char *reads, int read_len, uint64_t r0, uint6
 }
 
 int64_t dko_extract(const char *reads, size_t nbytes, int read_len, int k, int canonical,
-                    uint32_t num_pes, int32_t owner_bin, uint64_t *out, uint64_t cap) {
+                    uint32_t num_pes, int32_t owner_bin, uint64_t seed, uint64_t *out, uint64_t cap) {
     if (k < 1 || k > 64 || num_pes < 1) return -DKO_E_INVAL;
     if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return -DKO_E_INVAL;
     int64_t nrec = count_records(reads, nbytes, read_len);
     if (nrec < 0) return nrec;
-    emit_ctx c = {k, canonical, key_words_for(k), num_pes, owner_bin, out, cap, 0};
+    emit_ctx c = {k, canonical, key_words_for(k), num_pes, owner_bin, seed_or_default(seed), out, cap, 0};
     return extract_range(reads, read_len, 0, (uint64_t)nrec, &c);
 }
 
@@ -272,11 +282,12 @@ static uint64_t rle(uint64_t *a, uint64_t n, int kw, uint32_t *counts) {
 }
 
 int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonical,
-              uint32_t num_pes, int32_t owner_bin, uint64_t **keys_out, uint32_t **counts_out,
+              uint32_t num_pes, int32_t owner_bin, uint64_t seed, uint64_t **keys_out, uint32_t **counts_out,
               uint64_t *d_out, uint64_t *n_out, uint64_t *bin_offsets) {
     if (!keys_out || !counts_out || !d_out) return DKO_E_INVAL;
     *keys_out = NULL; *counts_out = NULL; *d_out = 0;
-    int64_t n = dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, NULL, 0);
+    seed = seed_or_default(seed);
+    int64_t n = dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, seed, NULL, 0);
     if (n < 0) return (int)-n;
     if ((uint64_t)n > 0xFFFFFFFFULL) return DKO_E_TOOBIG;
     int kw = key_words_for(k);
@@ -284,7 +295,7 @@ int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonic
     uint64_t *tmp = (uint64_t *)malloc(((size_t)n + 1) * kw * sizeof(uint64_t));
     uint32_t *counts = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
     if (!keys || !tmp || !counts) { free(keys); free(tmp); free(counts); return DKO_E_NOMEM; }
-    dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, keys, (uint64_t)n);
+    dko_extract(reads, nbytes, read_len, k, canonical, num_pes, owner_bin, seed, keys, (uint64_t)n);
     if (n_out) *n_out = (uint64_t)n;
     uint64_t d = 0;
     if (num_pes > 1 && owner_bin < 0) {
@@ -296,7 +307,7 @@ int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonic
         for (int64_t i = 0; i < n; i++) {
             u128 key = keys[i * kw];
             if (kw == 2) key |= (u128)keys[i * kw + 1] << 64;
-            own[i] = owner_of(key, kw, num_pes);
+            own[i] = owner_of(key, kw, num_pes, seed);
             cnt[own[i] + 1]++;
         }
         for (uint32_t p = 0; p < num_pes; p++) cnt[p + 1] += cnt[p];
@@ -332,33 +343,70 @@ int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonic
 
 /* -------------------------------------------------- threaded P-PE shape --- */
 
+/* Shared-memory stand-in for DAKC's P-PE run, used to time the CPU baseline
+ * and to check large GPU tables by checksum.  T workers each act as one PE:
+ * Phase 1 parses a block of records (fqreader.cpp:43-52) and routes every
+ * k-mer to its worker bin (kcounter.cpp:162-174 routes it to its owner PE);
+ * after a barrier, Phase 2 sorts and run-length reduces each bin
+ * (kcounter.cpp:385-411).  A worker bin is MurmurHash64A % T, the reference's
+ * owner_pe over T PEs, taken from (hash / P) when the job itself keeps only
+ * owner bin b of P (so the kept k-mers still spread over the workers). */
 typedef struct {
-    int tid, T, read_len, k, canonical;
+    int tid, T, kw;
+    uint64_t seed, div;          /* worker bin = (hash / div) % T */
+    /* Phase 1 input: records [r0, r1), or keys [k0, k1) of a key array */
     const char *reads;
+    int read_len, k, canonical;
+    uint32_t num_pes;
+    int32_t owner_bin;
     uint64_t r0, r1;
-    uint64_t *local;      /* this worker's k-mers (phase 1) */
-    uint64_t nlocal;
-    uint64_t *cnt;        /* T x T: cnt[t*T + o] */
-    uint64_t *global, *tmp;
-    uint64_t *bin_start;  /* T + 1 */
+    const uint64_t *keys_in;
+    uint64_t k0, k1;
+    uint64_t *local, nlocal;     /* this worker's k-mers (Phase 1), capacity then count */
+    uint64_t *cnt;               /* T x T: cnt[t*T + o] */
+    uint64_t *global, *tmp;      /* n keys each (kw words per key) */
+    uint64_t *bin_start;         /* T + 1 */
     uint64_t d, checksum;
-    int status;
     pthread_barrier_t *bar;
 } mt_arg;
 
+static inline u128 key_at(const uint64_t *a, uint64_t i, int kw) {
+    return kw == 1 ? (u128)a[i] : ((u128)a[2 * i] | ((u128)a[2 * i + 1] << 64));
+}
+static inline void key_put(uint64_t *a, uint64_t i, int kw, u128 x) {
+    if (kw == 1) {
+        a[i] = (uint64_t)x;
+    } else {
+        a[2 * i] = (uint64_t)x;
+        a[2 * i + 1] = (uint64_t)(x >> 64);
+    }
+}
+static inline uint32_t worker_bin(const mt_arg *a, u128 x) {
+    return (uint32_t)((key_hash(x, a->kw, a->seed) / a->div) % (uint64_t)a->T);
+}
+
 static void *mt_worker(void *p) {
     mt_arg *a = (mt_arg *)p;
-    int T = a->T;
-    /* Phase 1: parse this worker's block of records (fqreader.cpp:43-52) */
-    emit_ctx c = {a->k, a->canonical, 1, 1, -1, a->local, a->nlocal, 0};
-    extract_range(a->reads, a->read_len, a->r0, a->r1, &c);
-    a->nlocal = c.n;
+    const int T = a->T, kw = a->kw;
+    /* Phase 1: this worker's records (or its slice of the key array) */
+    if (a->reads) {
+        emit_ctx c = {a->k, a->canonical, kw, a->num_pes, a->owner_bin, a->seed, a->local, a->nlocal, 0};
+        extract_range(a->reads, a->read_len, a->r0, a->r1, &c);
+        a->nlocal = c.n;
+    } else {
+        a->nlocal = a->k1 - a->k0;
+        memcpy(a->local, a->keys_in + a->k0 * kw, (size_t)a->nlocal * kw * sizeof(uint64_t));
+    }
     uint64_t *mycnt = a->cnt + (size_t)a->tid * T;
     for (int o = 0; o < T; o++) mycnt[o] = 0;
-    for (uint64_t i = 0; i < c.n; i++) mycnt[dko_murmur64a_u64(a->local[i], DKO_OWNER_SEED) % (uint64_t)T]++;
+    for (uint64_t i = 0; i < a->nlocal; i++) mycnt[worker_bin(a, key_at(a->local, i, kw))]++;
     pthread_barrier_wait(a->bar);
-    /* owner bins: bin o holds worker 0's k-mers for o, then worker 1's, ... */
+    /* worker bins: bin o holds worker 0's k-mers for o, then worker 1's, ... */
     uint64_t *pos = (uint64_t *)malloc((size_t)T * sizeof(uint64_t));
+    if (!pos) {
+        a->d = 0;
+        return NULL;
+    }
     for (int o = 0; o < T; o++) {
         uint64_t s = 0;
         for (int o2 = 0; o2 < o; o2++) for (int t = 0; t < T; t++) s += a->cnt[(size_t)t * T + o2];
@@ -371,19 +419,20 @@ static void *mt_worker(void *p) {
         for (size_t i = 0; i < (size_t)T * T; i++) tot += a->cnt[i];
         a->bin_start[T] = tot;
     }
-    for (uint64_t i = 0; i < c.n; i++) {
-        uint64_t x = a->local[i];
-        a->global[pos[dko_murmur64a_u64(x, DKO_OWNER_SEED) % (uint64_t)T]++] = x;
+    for (uint64_t i = 0; i < a->nlocal; i++) {
+        const u128 x = key_at(a->local, i, kw);
+        key_put(a->global, pos[worker_bin(a, x)]++, kw, x);
     }
     free(pos);
     pthread_barrier_wait(a->bar);
-    /* Phase 2: each owner sorts and reduces its own bin (kcounter.cpp:385-411) */
-    uint64_t b0 = a->bin_start[a->tid], b1 = a->bin_start[a->tid + 1];
-    sort_u64(a->global + b0, a->tmp + b0, b1 - b0, 2 * a->k);
-    uint32_t *counts = (uint32_t *)(a->tmp + b0); /* reuse scratch */
-    uint64_t d = rle(a->global + b0, b1 - b0, 1, counts);
+    /* Phase 2: each worker sorts and reduces its own bin (kcounter.cpp:385-411) */
+    const uint64_t b0 = a->bin_start[a->tid], b1 = a->bin_start[a->tid + 1];
+    uint64_t *g = a->global + b0 * kw, *t = a->tmp + b0 * kw;
+    sort_keys(g, t, b1 - b0, a->k);
+    uint32_t *counts = (uint32_t *)t; /* reuse scratch */
+    const uint64_t d = rle(g, b1 - b0, kw, counts);
     a->d = d;
-    a->checksum = dko_table_checksum(a->global + b0, counts, d, 1);
+    a->checksum = dko_table_checksum(g, counts, d, kw);
     return NULL;
 }
 
@@ -393,17 +442,14 @@ static double now_s(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, int threads,
-                 uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s) {
-    if (threads < 1 || threads > 1024 || k < 1 || k > 32) return DKO_E_INVAL;
-    int64_t nrec = count_records(reads, nbytes, read_len);
-    if (nrec < 0) return (int)-nrec;
-    int T = threads;
-    uint64_t per_read = read_len >= k ? (uint64_t)(read_len - k + 1) : 0;
-    uint64_t nmax = (uint64_t)nrec * per_read;
-    uint64_t *local = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
-    uint64_t *global = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
-    uint64_t *tmp = (uint64_t *)malloc((size_t)(nmax + 1) * sizeof(uint64_t));
+/* Run the T workers over records or keys; nmax bounds the k-mers. */
+static int run_mt(mt_arg *proto, uint64_t nunits, uint64_t per_unit, uint64_t nmax, int threads,
+                  uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s) {
+    const int T = threads, kw = proto->kw;
+    const size_t kb = (size_t)kw * sizeof(uint64_t);
+    uint64_t *local = (uint64_t *)malloc((size_t)(nmax + 1) * kb);
+    uint64_t *global = (uint64_t *)malloc((size_t)(nmax + 1) * kb);
+    uint64_t *tmp = (uint64_t *)malloc((size_t)(nmax + 1) * kb);
     uint64_t *cnt = (uint64_t *)calloc((size_t)T * T, sizeof(uint64_t));
     uint64_t *bin_start = (uint64_t *)calloc((size_t)T + 1, sizeof(uint64_t));
     mt_arg *args = (mt_arg *)calloc((size_t)T, sizeof(mt_arg));
@@ -413,20 +459,27 @@ int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int cano
         return DKO_E_NOMEM;
     }
     /* touch the buffers once so page faults are not timed */
-    memset(local, 0, (size_t)(nmax + 1) * 8);
-    memset(global, 0, (size_t)(nmax + 1) * 8);
-    memset(tmp, 0, (size_t)(nmax + 1) * 8);
+    memset(local, 0, (size_t)(nmax + 1) * kb);
+    memset(global, 0, (size_t)(nmax + 1) * kb);
+    memset(tmp, 0, (size_t)(nmax + 1) * kb);
     pthread_barrier_t bar;
     pthread_barrier_init(&bar, NULL, (unsigned)T);
-    double t0 = now_s();
+    const double t0 = now_s();
     for (int t = 0; t < T; t++) {
         mt_arg *a = &args[t];
-        a->tid = t; a->T = T; a->read_len = read_len; a->k = k; a->canonical = canonical != 0;
-        a->reads = reads;
-        a->r0 = (uint64_t)nrec * t / T;
-        a->r1 = (uint64_t)nrec * (t + 1) / T;
-        a->local = local + a->r0 * per_read;
-        a->nlocal = (a->r1 - a->r0) * per_read;
+        *a = *proto;
+        a->tid = t;
+        a->T = T;
+        const uint64_t u0 = nunits * (uint64_t)t / T, u1 = nunits * (uint64_t)(t + 1) / T;
+        if (a->reads) {
+            a->r0 = u0;
+            a->r1 = u1;
+        } else {
+            a->k0 = u0;
+            a->k1 = u1;
+        }
+        a->local = local + u0 * per_unit * kw;
+        a->nlocal = (u1 - u0) * per_unit;
         a->cnt = cnt; a->global = global; a->tmp = tmp; a->bin_start = bin_start; a->bar = &bar;
         pthread_create(&th[t], NULL, mt_worker, a);
     }
@@ -436,7 +489,7 @@ int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int cano
         d += args[t].d;
         cs += args[t].checksum;
     }
-    double t1 = now_s();
+    const double t1 = now_s();
     pthread_barrier_destroy(&bar);
     if (n_out) *n_out = bin_start[T];
     if (d_out) *d_out = d;
@@ -444,6 +497,93 @@ int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int cano
     if (elapsed_s) *elapsed_s = t1 - t0;
     free(local); free(global); free(tmp); free(cnt); free(bin_start); free(args); free(th);
     return DKO_OK;
+}
+
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                 int32_t owner_bin, uint64_t seed, int threads, uint64_t *n_out, uint64_t *d_out,
+                 uint64_t *checksum_out, double *elapsed_s) {
+    if (threads < 1 || threads > 1024 || k < 1 || k > 64 || num_pes < 1) return DKO_E_INVAL;
+    if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return DKO_E_INVAL;
+    int64_t nrec = count_records(reads, nbytes, read_len);
+    if (nrec < 0) return (int)-nrec;
+    const uint64_t per_read = read_len >= k ? (uint64_t)(read_len - k + 1) : 0;
+    mt_arg proto;
+    memset(&proto, 0, sizeof(proto));
+    proto.kw = key_words_for(k);
+    proto.seed = seed_or_default(seed);
+    proto.div = num_pes > 1 && owner_bin >= 0 ? num_pes : 1;
+    proto.reads = nrec ? reads : "";
+    proto.read_len = read_len;
+    proto.k = k;
+    proto.canonical = canonical != 0;
+    proto.num_pes = num_pes;
+    proto.owner_bin = num_pes > 1 ? owner_bin : -1;
+    return run_mt(&proto, (uint64_t)nrec, per_read, (uint64_t)nrec * per_read, threads, n_out, d_out,
+                  checksum_out, elapsed_s);
+}
+
+int dko_count_keys_mt(const uint64_t *keys, uint64_t n, int k, int threads, uint64_t *d_out,
+                      uint64_t *checksum_out, double *elapsed_s) {
+    if (threads < 1 || threads > 1024 || k < 1 || k > 64 || (n && !keys)) return DKO_E_INVAL;
+    mt_arg proto;
+    memset(&proto, 0, sizeof(proto));
+    proto.kw = key_words_for(k);
+    proto.seed = DKO_OWNER_SEED;
+    proto.div = 1;
+    proto.k = k;
+    proto.keys_in = keys;
+    uint64_t nn = 0;
+    return run_mt(&proto, n, 1, n, threads, &nn, d_out, checksum_out, elapsed_s);
+}
+
+typedef struct {
+    emit_ctx c;
+    const char *reads;
+    int read_len;
+    uint64_t r0, r1;
+} ex_arg;
+
+static void *ex_worker(void *p) {
+    ex_arg *a = (ex_arg *)p;
+    extract_range(a->reads, a->read_len, a->r0, a->r1, &a->c);
+    return NULL;
+}
+
+int64_t dko_extract_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                       int32_t owner_bin, uint64_t seed, int threads, uint64_t *out, uint64_t cap) {
+    if (threads < 1 || threads > 1024 || k < 1 || k > 64 || num_pes < 1) return -DKO_E_INVAL;
+    if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return -DKO_E_INVAL;
+    int64_t nrec = count_records(reads, nbytes, read_len);
+    if (nrec < 0) return nrec;
+    const int kw = key_words_for(k), T = threads;
+    const uint64_t per_read = read_len >= k ? (uint64_t)(read_len - k + 1) : 0;
+    ex_arg *args = (ex_arg *)calloc((size_t)T, sizeof(ex_arg));
+    pthread_t *th = (pthread_t *)calloc((size_t)T, sizeof(pthread_t));
+    uint64_t *scratch = (uint64_t *)malloc(((size_t)nrec * per_read + 1) * kw * sizeof(uint64_t));
+    if (!args || !th || !scratch) {
+        free(args); free(th); free(scratch);
+        return -DKO_E_NOMEM;
+    }
+    for (int t = 0; t < T; t++) {
+        ex_arg *a = &args[t];
+        a->reads = reads;
+        a->read_len = read_len;
+        a->r0 = (uint64_t)nrec * t / T;
+        a->r1 = (uint64_t)nrec * (t + 1) / T;
+        emit_ctx c = {k, canonical, kw, num_pes, num_pes > 1 ? owner_bin : -1, seed_or_default(seed),
+                      scratch + a->r0 * per_read * kw, (a->r1 - a->r0) * per_read, 0};
+        a->c = c;
+        pthread_create(&th[t], NULL, ex_worker, a);
+    }
+    uint64_t o = 0;
+    for (int t = 0; t < T; t++) {
+        pthread_join(th[t], NULL);
+        const uint64_t n = args[t].c.n;
+        if (out && o + n <= cap) memcpy(out + o * kw, args[t].c.out, (size_t)n * kw * sizeof(uint64_t));
+        o += n;
+    }
+    free(args); free(th); free(scratch);
+    return (int64_t)o;
 }
 
 /* ------------------------------------------------------- after the count --- */
@@ -497,14 +637,14 @@ static int key_cmp(const uint64_t *a, const uint64_t *b, int kw) {
  * bin_offsets is given, else in the whole table: the lookup kcounter.cpp:381-383
  * describes for the sorted lightdbg. */
 void dko_query(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words,
-               const uint64_t *bin_offsets, uint32_t num_bins, const uint64_t *queries, uint64_t nq,
-               uint32_t *out) {
+               const uint64_t *bin_offsets, uint32_t num_bins, uint64_t seed, const uint64_t *queries,
+               uint64_t nq, uint32_t *out) {
+    seed = seed_or_default(seed);
     for (uint64_t i = 0; i < nq; i++) {
         const uint64_t *q = queries + i * key_words;
         uint64_t lo = 0, hi = d;
         if (bin_offsets && num_bins) {
-            uint64_t h = key_words == 1 ? dko_murmur64a_u64(q[0], DKO_OWNER_SEED)
-                                        : dko_murmur64a_u128(q[0], q[1], DKO_OWNER_SEED);
+            uint64_t h = key_words == 1 ? dko_murmur64a_u64(q[0], seed) : dko_murmur64a_u128(q[0], q[1], seed);
             uint32_t o = (uint32_t)(h % num_bins);
             lo = bin_offsets[o];
             hi = bin_offsets[o + 1];

@@ -36,7 +36,8 @@ extern "C" {
 #define DKO_E_NOMEM 3
 #define DKO_E_TOOBIG 4
 
-/* kcounter.cpp:52 -- the fixed owner-hash seed */
+/* kcounter.cpp:52 -- the reference's owner-hash seed; a seed argument of 0
+ * stands for it */
 #define DKO_OWNER_SEED 0x9E3779B97F4A7C15ULL
 
 /* common.hpp:72-92: ASCII -> 2-bit code (C0 A1 T2 G3), 0xF0 for M/m, 0xFF otherwise */
@@ -55,7 +56,7 @@ uint64_t dko_murmur64a_u128(uint64_t lo, uint64_t hi, uint64_t seed);
  * owner_bin are kept when num_pes > 1 and owner_bin >= 0.
  * Returns the k-mer count, or -(status) on error.  out may be NULL (count only). */
 int64_t dko_extract(const char *reads, size_t nbytes, int read_len, int k, int canonical,
-                    uint32_t num_pes, int32_t owner_bin, uint64_t *out, uint64_t cap);
+                    uint32_t num_pes, int32_t owner_bin, uint64_t seed, uint64_t *out, uint64_t cap);
 
 /* Full single-PE count (kcounter.cpp:247-414, HITTER=0): extract, sort ascending
  * as unsigned, run-length reduce.  With num_pes > 1 and owner_bin == -1 the
@@ -64,20 +65,35 @@ int64_t dko_extract(const char *reads, size_t nbytes, int read_len, int k, int c
  * *keys_out (d * key_words u64) and *counts_out (d u32) are malloc'd; free with
  * dko_free.  Returns a status. */
 int dko_count(const char *reads, size_t nbytes, int read_len, int k, int canonical,
-              uint32_t num_pes, int32_t owner_bin, uint64_t **keys_out, uint32_t **counts_out,
+              uint32_t num_pes, int32_t owner_bin, uint64_t seed, uint64_t **keys_out, uint32_t **counts_out,
               uint64_t *d_out, uint64_t *n_out, uint64_t *bin_offsets);
 
-/* DAKC's P-PE shape on shared memory, for timing the CPU baseline:
- * `threads` workers each parse 1/threads of the records (fqreader.cpp:43-52),
- * scatter k-mers to threads owner bins by MurmurHash64A % threads
+/* DAKC's P-PE shape on shared memory, for timing the CPU baseline and checking
+ * large tables: `threads` workers each parse 1/threads of the records
+ * (fqreader.cpp:43-52), route k-mers to threads worker bins by MurmurHash64A
  * (kcounter.cpp:50-54, 162-174), barrier, then each sorts and run-length
- * reduces its own bin (kcounter.cpp:385-411).  Returns the total k-mer count,
- * distinct count and an order-independent checksum of the (kmer, count)
- * multiset (dko_table_checksum) so the result can be compared with any table.
+ * reduces its own bin (kcounter.cpp:385-411).  With num_pes > 1 and
+ * owner_bin >= 0 only owner bin owner_bin's k-mers are counted.  Returns the
+ * total k-mer count, distinct count and an order-independent checksum of the
+ * (kmer, count) multiset (dko_table_checksum), so the result can be compared
+ * with any table of the same entries (owner-bin order does not change it).
  * *elapsed_s is the wall time of the counting phases (file read excluded, as
- * kcounter.cpp:262). */
-int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, int threads,
-                 uint64_t *n_out, uint64_t *d_out, uint64_t *checksum_out, double *elapsed_s);
+ * kcounter.cpp:262).  k <= 64. */
+int dko_count_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                 int32_t owner_bin, uint64_t seed, int threads, uint64_t *n_out, uint64_t *d_out,
+                 uint64_t *checksum_out, double *elapsed_s);
+
+/* The same Phase 2 over an array of n keys (kw words each, as dko_extract
+ * writes them): the owner's sort and run-length reduce of its received
+ * k-mers (kcounter.cpp:385-411).  Returns distinct count and checksum. */
+int dko_count_keys_mt(const uint64_t *keys, uint64_t n, int k, int threads, uint64_t *d_out,
+                      uint64_t *checksum_out, double *elapsed_s);
+
+/* dko_extract on `threads` workers, keys in record order (out may be NULL:
+ * count only; at most cap keys are written).  Returns the k-mer count or
+ * -(status). */
+int64_t dko_extract_mt(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                       int32_t owner_bin, uint64_t seed, int threads, uint64_t *out, uint64_t cap);
 
 /* order-independent checksum of a table: sum over entries of mix(key words, count) */
 uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words);
@@ -93,8 +109,8 @@ uint32_t dko_histogram(const uint32_t *counts, uint64_t d, uint32_t num_buckets,
 uint64_t dko_filter(uint64_t *keys, uint32_t *counts, uint64_t d, int key_words, uint32_t min_count,
                     uint64_t *bin_offsets, uint32_t num_bins, uint64_t *total_out);
 void dko_query(const uint64_t *keys, const uint32_t *counts, uint64_t d, int key_words,
-               const uint64_t *bin_offsets, uint32_t num_bins, const uint64_t *queries, uint64_t nq,
-               uint32_t *out);
+               const uint64_t *bin_offsets, uint32_t num_bins, uint64_t seed, const uint64_t *queries,
+               uint64_t nq, uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -89,7 +89,7 @@ def value(s: str) -> int:
 
 
 def kmers(buf: bytes, read_len: int, k: int, canonical: bool = False,
-          num_pes: int = 1, owner_bin: int = -1) -> list[int]:
+          num_pes: int = 1, owner_bin: int = -1, seed: int = OWNER_SEED) -> list[int]:
     out = []
     for rec in records(buf, read_len):
         s = rec.decode("latin-1")
@@ -105,23 +105,23 @@ def kmers(buf: bytes, read_len: int, k: int, canonical: bool = False,
                 v = value(sub)
                 if canonical:
                     v = min(v, value(sub[::-1].translate(COMP)))
-                if num_pes > 1 and owner_bin >= 0 and owner(v, k, num_pes) != owner_bin:
+                if num_pes > 1 and owner_bin >= 0 and owner(v, k, num_pes, seed) != owner_bin:
                     continue
                 out.append(v)
     return out
 
 
-def owner(v: int, k: int, num_pes: int) -> int:
+def owner(v: int, k: int, num_pes: int, seed: int = OWNER_SEED) -> int:
     nbytes = 8 if k <= 32 else 16
-    return murmur64a(v.to_bytes(nbytes, "little"), OWNER_SEED) % num_pes
+    return murmur64a(v.to_bytes(nbytes, "little"), seed) % num_pes
 
 
 def count(buf: bytes, read_len: int, k: int, canonical: bool = False,
-          num_pes: int = 1, owner_bin: int = -1) -> list[tuple[int, int]]:
+          num_pes: int = 1, owner_bin: int = -1, seed: int = OWNER_SEED) -> list[tuple[int, int]]:
     """Sorted (kmer, count); with all P bins, ordered by (owner, kmer)."""
-    c = Counter(kmers(buf, read_len, k, canonical, num_pes, owner_bin))
+    c = Counter(kmers(buf, read_len, k, canonical, num_pes, owner_bin, seed))
     if num_pes > 1 and owner_bin < 0:
-        return sorted(c.items(), key=lambda kv: (owner(kv[0], k, num_pes), kv[0]))
+        return sorted(c.items(), key=lambda kv: (owner(kv[0], k, num_pes, seed), kv[0]))
     return sorted(c.items())
 
 

@@ -32,9 +32,9 @@ def assert_same(gpu, ora):
         assert np.array_equal(gpu.bin_offsets, offs)
 
 
-def run_both(kc, data, L, k, canonical=False, P=1, b=-1):
-    g = kc.count(data, k=k, read_len=L, canonical=canonical, num_pes=P, owner_bin=b)
-    o = oracle.count(data, L, k, canonical, P, b)
+def run_both(kc, data, L, k, canonical=False, P=1, b=-1, seed=0):
+    g = kc.count(data, k=k, read_len=L, canonical=canonical, num_pes=P, owner_bin=b, hash_seed=seed)
+    o = oracle.count(data, L, k, canonical, P, b, seed)
     assert_same(g, o)
     return g
 
@@ -135,6 +135,29 @@ def test_owner_bins_k31(kc):
     data = _lib.generate_reads(p)
     for P, b in [(64, 0), (64, 63), (64, -1), (7, -1), (256, -1), (1000, 3)]:
         run_both(kc, data, 150, 31, P=P, b=b)
+
+
+def test_owner_hash_seed(kc):
+    # a named owner seed moves k-mers between bins the same way on the GPU as
+    # in the oracle; 0 is the reference's seed; queries find keys in the
+    # seeded bins
+    p = _lib.gen_params(14, 30_000, 150, genome_len=300_000, sub_rate=0.005)
+    data = _lib.generate_reads(p)
+    seed = 0xD1B54A32D192ED03
+    for P, b in [(64, 5), (64, -1), (7, -1)]:
+        g = run_both(kc, data, 150, 31, P=P, b=b, seed=seed)
+        run_both(kc, data, 150, 55, P=P, b=b, seed=seed)
+    assert not np.array_equal(g.bin_offsets, run_both(kc, data, 150, 31, P=7, b=-1).bin_offsets)
+    d = kc.device_alloc(data.size)
+    try:
+        kc.h2d(d, data)
+        t = kc.count_device(d, data.size, num_pes=7, owner_bin=-1, hash_seed=seed)
+        assert t.hash_seed == seed
+        keys, counts, _, offs = oracle.count(data, 150, 31, False, 7, -1, seed)
+        q = np.concatenate([keys[::97, 0], np.arange(1000, dtype=np.uint64)])
+        assert np.array_equal(kc.table_query(t, q), oracle.query(keys, counts, q, offs, seed))
+    finally:
+        kc.device_free(d)
 
 
 def test_runs_longer_than_a_tile(kc):

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_status_strings():
     lib = _lib.load()
-    assert lib.dakc_abi_version() == 1
+    assert lib.dakc_abi_version() == 2
     for s in range(11):
         assert _lib.status_str(s)
 
@@ -89,3 +89,27 @@ def test_kcounter_reports_missing_gpu_cleanly():
     # on a host without a GPU, context creation fails with a status, not a crash
     from dakc_amd.kcounter import gpu_available
     assert gpu_available() in (True, False)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    # the ctypes layouts of the binding are the C layouts of include/dakc.h
+    structs = {"dakc_job": _lib.Job, "dakc_table": _lib.Table, "dakc_opts": _lib.Opts,
+               "dakc_stats": _lib.Stats, "dakc_serve_stats": _lib.ServeStats,
+               "dakc_gen_params": _lib.GenParams}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dakc.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

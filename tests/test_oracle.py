@@ -184,3 +184,57 @@ def test_threaded_shape_matches_single_pe():
             n2, d2, cs, el = oracle.count_mt(data, 150, 31, T, canonical=canonical)
             assert (n2, d2) == (n, len(counts))
             assert cs == oracle.checksum(keys, counts)
+
+
+@pytest.mark.parametrize("k", [31, 63])
+def test_threaded_shape_two_word_keys_and_owner_bins(k):
+    # the threaded count of one owner bin (and of all bins) equals the single-PE
+    # table of that bin; the all-bins checksum is the P=1 table's (each k-mer
+    # has one owner)
+    L = 250
+    data = random_reads(13, 600, L, alphabet="ACGTACGTACGTN", mpad=False)
+    seed = 0x1234567
+    fk, fc, fn, _ = oracle.count(data, L, k)
+    for T in (1, 5):
+        n2, d2, cs, _ = oracle.count_mt(data, L, k, T, num_pes=64, owner_bin=-1)
+        assert (n2, d2, cs) == (fn, len(fc), oracle.checksum(fk, fc))
+        for b in (0, 17):
+            bk, bc, bn, _ = oracle.count(data, L, k, num_pes=64, owner_bin=b, seed=seed)
+            n3, d3, cs3, _ = oracle.count_mt(data, L, k, T, num_pes=64, owner_bin=b, seed=seed)
+            assert (n3, d3, cs3) == (bn, len(bc), oracle.checksum(bk, bc))
+
+
+@pytest.mark.parametrize("k", [21, 55])
+def test_key_array_count_and_threaded_extract(k):
+    L = 150
+    data = random_reads(17, 900, L, alphabet="ACGTACGTACGTN", mpad=True)
+    keys = oracle.extract(data, L, k, canonical=True)
+    for T in (1, 4, 7):
+        assert np.array_equal(oracle.extract_mt(data, L, k, T, canonical=True), keys)
+    # counting the extracted keys is counting the reads
+    fk, fc, fn, _ = oracle.count(data, L, k, canonical=True)
+    assert fn == keys.shape[0]
+    for T in (1, 6):
+        d, cs, _ = oracle.count_keys_mt(keys, k, T)
+        assert (d, cs) == (len(fc), oracle.checksum(fk, fc))
+    d, cs, _ = oracle.count_keys_mt(np.zeros((0, keys.shape[1]), np.uint64), k, 3)
+    assert (d, cs) == (0, 0)
+
+
+def test_owner_seed_matches_bruteforce():
+    # a named seed changes which bin a k-mer belongs to; 0 stands for the
+    # reference's seed (kcounter.cpp:52)
+    data = random_reads(19, 40, 60, alphabet="ACGTACGTACGTACGTN", mpad=False)
+    for k in (15, 40):
+        for seed in (0x5EED, 2**64 - 1):
+            for b in (0, 3):
+                keys, counts, n, _ = oracle.count(data, 60, k, num_pes=5, owner_bin=b, seed=seed)
+                want = brute.count(data, 60, k, num_pes=5, owner_bin=b, seed=seed)
+                got = [(int(r[0]) | (int(r[1]) << 64) if keys.shape[1] == 2 else int(r[0]), int(c))
+                       for r, c in zip(keys, counts)]
+                assert got == want
+        a = oracle.count(data, 60, k, num_pes=5, owner_bin=-1)
+        z = oracle.count(data, 60, k, num_pes=5, owner_bin=-1, seed=oracle.OWNER_SEED)
+        o = oracle.count(data, 60, k, num_pes=5, owner_bin=-1, seed=0x5EED)
+        assert np.array_equal(a[0], z[0]) and np.array_equal(a[3], z[3])
+        assert a[3][-1] > 50 and not np.array_equal(a[3], o[3])

@@ -673,8 +673,7 @@ int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, 
   uint32_t e;
   std::memcpy(nd, m, 16);
   std::memcpy(&e, m + 16, 4);
-  if (e & (ERR_SEPARATOR | ERR_NUL_RECORD)) return DAKC_E_FORMAT;
-  if (e & ERR_OVERFLOW) return STATUS_MSD_OVERFLOW;
+  if (int s = status_of_err(e)) return s;
   *n = nd[0];
   *d = b.plans[j].nmax ? nd[1] : 0;
   return DAKC_OK;
@@ -757,8 +756,7 @@ int finish_count(JobWs &ws, const Plan &pl, uint64_t *n, uint64_t *d) {
   uint32_t e = 0;
   std::memcpy(nd, ws.hmeta, 16);
   std::memcpy(&e, ws.hmeta + 16, 4);
-  if (e & (ERR_SEPARATOR | ERR_NUL_RECORD)) return DAKC_E_FORMAT;
-  if (e & ERR_OVERFLOW) return STATUS_MSD_OVERFLOW;
+  if (int s = status_of_err(e)) return s;
   if (pl.nmax == 0) nd[1] = 0;
   *n = nd[0];
   *d = nd[1];

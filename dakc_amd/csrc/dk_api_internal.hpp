@@ -73,9 +73,6 @@ struct Plan {
   uint32_t tune_fin_nt = 1024, tune_fill_pct = 90, num_cus = 256;
 };
 
-// internal status: an MSD sub-bucket did not fit in LDS; rerun on the LSD path
-constexpr int STATUS_MSD_OVERFLOW = -100;
-
 // path choice (dakc_opts.algo)
 int plan_choose_path(Plan *pl, int algo);
 

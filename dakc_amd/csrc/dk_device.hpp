@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dk_status.hpp"
+
 namespace dk {
 
 struct K2 {
@@ -171,18 +173,32 @@ DK_DEV void lb_store(uint64_t *p, uint64_t w) {
 DK_DEV uint64_t lb_load(uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// A look-back wait gives up after this many polls (several seconds, far
+// beyond any predecessor's time to publish): it sets ERR_STALL in the job's
+// error word and carries on as if the predecessor had published an inclusive
+// prefix of 0.  Prefixes can then only come out too small, so every write
+// stays inside the buffers, the successors find a published value and end,
+// and the host reports DAKC_E_DEVICE instead of the GPU hanging.
+constexpr uint32_t LB_SPIN_MAX = 1u << 22;
+
 // Exclusive prefix of `slot` over tiles [0, tile) with stride `stride` words
 // between consecutive tiles.  Tiles are taken in ticket order, so every
-// predecessor is already resident and will publish: the wait always ends.
+// predecessor is already resident and will publish: the wait ends, and is
+// bounded by LB_SPIN_MAX polls besides.
 DK_DEV uint64_t lb_lookback(uint64_t *flags, uint32_t tile, uint32_t stride, uint32_t slot,
-                            uint32_t epoch) {
+                            uint32_t epoch, uint32_t *err) {
   uint64_t excl = 0;
   int64_t t = (int64_t)tile - 1;
+  uint32_t polls = 0;
   while (t >= 0) {
     uint64_t w = lb_load(flags + (uint64_t)t * stride + slot);
     uint32_t e = (uint32_t)(w >> 50);
     uint64_t st = (w >> 48) & 3;
     if (e != epoch || st == 0) {
+      if (++polls > LB_SPIN_MAX) {
+        atomicOr(err, ERR_STALL);
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }

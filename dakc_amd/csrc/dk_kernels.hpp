@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dk_status.hpp"
+
 namespace dk {
 
 // Fixed-width record input (fqreader.cpp:43, kcounter_funcs.cpp:102): each
@@ -26,8 +28,7 @@ struct ReadsDesc {
   uint64_t seed;         // owner-hash seed
 };
 
-// error bits reported through the job's err word
-enum : uint32_t { ERR_SEPARATOR = 1u, ERR_NUL_RECORD = 2u, ERR_OVERFLOW = 4u };
+// error bits reported through the job's err word: dk_status.hpp
 
 // Which digit of a key a pass sorts or partitions by.
 //   DIG_BITS : bits [shift, shift + nbits) of the 2k-bit k-mer value

@@ -64,6 +64,7 @@ struct LinesArgs {
   uint64_t *nl;     // newline byte offsets, in order
   uint64_t *flags;  // look-back words, one per tile, zeroed
   uint32_t *ticket;
+  uint32_t *err;    // ERR_STALL when a look-back wait gave up
 };
 
 // Newline offsets in byte order: tiles in ticket order, each thread reading
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(LN_NT) void k_find_nl(LinesArgs a) {
       s_excl = 0;
     } else {
       lb_store(a.flags + tile, lb_pack(epoch, LB_AGG, run));
-      const uint64_t ex = lb_lookback(a.flags, tile, 1, 0, epoch);
+      const uint64_t ex = lb_lookback(a.flags, tile, 1, 0, epoch, a.err);
       lb_store(a.flags + tile, lb_pack(epoch, LB_INC, ex + run));
       s_excl = ex;
     }
@@ -189,8 +190,10 @@ int dk::convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int forma
   if (ntiles > 0xFFFFFFFFull) return DAKC_E_TOOBIG;
   if ((s = c->post_a.ensure(ntiles * 8 + 64, st, false))) return s;
   uint64_t *flags = (uint64_t *)c->post_a.p;
-  auto *meta = (unsigned long long *)((uint8_t *)c->post_a.p + ntiles * 8);  // [0] total nl, [1] max len
+  // [0] total nl, [1] max len, [2] ticket, [3] error word
+  auto *meta = (unsigned long long *)((uint8_t *)c->post_a.p + ntiles * 8);
   uint32_t *ticket = (uint32_t *)(meta + 2);
+  uint32_t *err = (uint32_t *)(meta + 3);
   launch_fill_u64((uint64_t *)c->post_a.p, 0, ntiles + 8, st);
   if (nbytes) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((nbytes + LN_NT * 64 - 1) / (LN_NT * 64),
@@ -204,7 +207,7 @@ int dk::convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int forma
       hipStreamSynchronize(st) != hipSuccess)
     return DAKC_E_DEVICE;
   if ((s = c->conv_nl.ensure(std::max<uint64_t>(nnl, 1) * 8, st, false))) return s;
-  LinesArgs la{in, nbytes, (uint64_t *)c->conv_nl.p, flags, ticket};
+  LinesArgs la{in, nbytes, (uint64_t *)c->conv_nl.p, flags, ticket, err};
   if (nbytes) k_find_nl<<<(uint32_t)ntiles, LN_NT, 0, st>>>(la);
   // lines: one per newline, plus a last one without its '\n'
   const uint64_t nlines = nnl + (nbytes && last != '\n' ? 1 : 0);
@@ -218,14 +221,16 @@ int dk::convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int forma
   ea.nrec = format == DAKC_FMT_FASTQ ? (nlines + 2) / 4 : nlines;  // lines 1, 5, 9, ... (NR % 4 == 2)
   ea.maxlen = (unsigned int *)(meta + 1);
   ea.out = nullptr;  // set once the output buffer is known
-  unsigned int maxlen = 0;
   if (ea.nrec) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((ea.nrec + 255) / 256, (uint64_t)c->num_cus * 8);
     k_max_line<<<grid, 256, 0, st>>>(ea);
-    if (hipMemcpyAsync(&maxlen, ea.maxlen, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return DAKC_E_DEVICE;
   }
+  unsigned long long m3[3];  // max len, ticket, error word
+  if (hipMemcpyAsync(m3, meta + 1, 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return DAKC_E_DEVICE;
+  if ((s = status_of_err((uint32_t)m3[2]))) return s;
+  const unsigned int maxlen = ea.nrec ? (unsigned int)m3[0] : 0u;
   if (read_len && maxlen > read_len) return DAKC_E_FORMAT;  // a read longer than the record width
   if (!read_len && maxlen > (unsigned int)MAX_READ_LEN) return DAKC_E_TOOBIG;
   // the longest read (at least 1: empty reads become "M\n", which hold no k-mer)

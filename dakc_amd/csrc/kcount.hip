@@ -302,6 +302,7 @@ struct SweepArgs {
   DigitSpec dig;
   uint32_t epoch;
   uint64_t key_cap;  // keys the in / out buffers hold
+  uint32_t *err;     // the job's error word
 };
 
 template <typename KeyT, int NT, int ITEMS, bool FROM_READS>
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
       lb_store(fl + tid, lb_pack(a.epoch, LB_INC, cnt));
     } else {
       lb_store(fl + tid, lb_pack(a.epoch, LB_AGG, cnt));
-      excl = lb_lookback(a.flags, tile, 256, tid, a.epoch);
+      excl = lb_lookback(a.flags, tile, 256, tid, a.epoch, a.err);
       lb_store(fl + tid, lb_pack(a.epoch, LB_INC, excl + cnt));
     }
     s_gbase[tid] = a.digit_start[tid] + excl;
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepArgs a) {
 template <typename KeyT, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint32_t *out_counts,
                                             uint64_t *n_dev, uint64_t *flags, uint32_t *ticket,
-                                            uint32_t epoch, uint64_t key_cap) {
+                                            uint32_t epoch, uint64_t key_cap, uint32_t *err) {
   constexpr int TILE = NT * ITEMS;
   __shared__ KeyT s_k[TILE];
   __shared__ uint16_t s_hp[TILE + 1];
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(NT) void k_rle(const KeyT *in, KeyT *out_keys, uint
       lb_store(flags, lb_pack(epoch, LB_INC, H));
     } else {
       lb_store(flags + tile, lb_pack(epoch, LB_AGG, H));
-      excl = lb_lookback(flags, tile, 1, 0, epoch);
+      excl = lb_lookback(flags, tile, 1, 0, epoch, err);
       lb_store(flags + tile, lb_pack(epoch, LB_INC, excl + H));
     }
     s_excl = excl;
@@ -1174,7 +1175,11 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
 
 // Exclusive prefix of sub-bucket `idx` over all earlier ones, one wavefront
 // reading 64 predecessors' flags at a time.  Called by a whole wavefront.
-DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) {
+// Bounded like lb_lookback: the wave polls its 64 predecessors together (one
+// poll count for the wave, in a scalar register); when it gives up, the lanes
+// still waiting take their predecessor as published with an inclusive prefix
+// of 0 and ERR_STALL is set.
+DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch, uint32_t *err) {
   const uint32_t lane = __lane_id();
   uint64_t excl = 0;
   int64_t base = (int64_t)idx - 1;
@@ -1182,16 +1187,26 @@ DK_DEV uint64_t lb_lookback_wave(uint64_t *flags, uint32_t idx, uint32_t epoch) 
     const int64_t t = base - (int64_t)lane;
     uint64_t val = 0;
     uint32_t st = LB_INC;
-    if (t >= 0) {
-      for (;;) {
+    bool ready = t < 0;
+    for (uint32_t polls = 0;; ++polls) {
+      if (!ready) {
         const uint64_t w = lb_load(flags + t);
         st = (uint32_t)((w >> 48) & 3);
         if ((uint32_t)(w >> 50) == epoch && st != 0) {
           val = w & 0xFFFFFFFFFFFFULL;
-          break;
+          ready = true;
         }
-        __builtin_amdgcn_s_sleep(1);
       }
+      if (!__ballot(!ready)) break;
+      if (polls >= LB_SPIN_MAX) {
+        if (!ready) {
+          atomicOr(err, ERR_STALL);
+          st = LB_INC;
+          val = 0;
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
     }
     const uint64_t inc = __ballot(st == LB_INC);
     const uint32_t first = inc ? (uint32_t)__ffsll((unsigned long long)inc) - 1 : 64u;
@@ -1214,14 +1229,14 @@ DK_DEV uint32_t fin_bin(const MsdArgs &m, const FinItem &f, const KeyT &x) {
 }
 
 DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t H, uint64_t *n_dev,
-                           uint64_t *s_excl) {
+                           uint32_t *err, uint64_t *s_excl) {
   constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
   const uint32_t tid = opaque_tid();
   if ((tid >> 6) == 0) {
     if (tid == 0) lb_store(flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));
     uint64_t excl = 0;
     if (sb > 0) {
-      excl = lb_lookback_wave(flags, sb, epoch);
+      excl = lb_lookback_wave(flags, sb, epoch, err);
       if (tid == 0) lb_store(flags + sb, lb_pack(epoch, LB_INC, excl + H));
     }
     if (tid == 0) {
@@ -1354,7 +1369,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   }
   if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
   // the bins are contiguous, so a wave's 64 pairs land in a few lines
-  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, s_excl);
+  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, J.err, s_excl);
   uint64_t *out_keys = reinterpret_cast<uint64_t *>(J.out1);
 #pragma unroll
   for (int i = 0; i < SPT; ++i)
@@ -1535,7 +1550,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   #pragma unroll
     for (int q = 0; q < ITEMS; ++q)
       if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
-    const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, &s_excl);
+    const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, err, &s_excl);
     for (uint32_t h = tid; h < H; h += NT) {
       const uint32_t i = s_hp[h];
       const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;
@@ -1667,6 +1682,7 @@ void launch_sweep(int kw, bool from_reads, const ReadsDesc &rd, const void *in, 
   a.dig = dig;
   a.epoch = epoch;
   a.key_cap = ws.key_cap;
+  a.err = ws.err;
   if (!grid) return;
   if (kw == 1) {
     if (from_reads) k_sweep<uint64_t, SWEEP_NT, SWEEP_ITEMS_1, true><<<grid, SWEEP_NT, 0, st>>>(a);
@@ -1684,11 +1700,11 @@ void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws,
   if (kw == 1)
     k_rle<uint64_t, SWEEP_NT, SWEEP_ITEMS_1><<<grid, SWEEP_NT, 0, st>>>(
         (const uint64_t *)sorted, (uint64_t *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket,
-        epoch, ws.key_cap);
+        epoch, ws.key_cap, ws.err);
   else
     k_rle<K2, SWEEP_NT, SWEEP_ITEMS_2><<<grid, SWEEP_NT, 0, st>>>(
         (const K2 *)sorted, (K2 *)out_keys, ws.counts, ws.n_dev, ws.rle_flags, ticket, epoch,
-        ws.key_cap);
+        ws.key_cap, ws.err);
 }
 
 void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes,

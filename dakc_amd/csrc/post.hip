@@ -92,6 +92,7 @@ struct FilterArgs {
   uint64_t *flags;          // look-back words, one per tile, zeroed
   uint32_t *ticket;
   unsigned long long *out;  // [0] kept entries, [1] sum of kept counts
+  uint32_t *err;            // ERR_STALL when a look-back wait gave up
 };
 
 // W: 64-bit words per key (held as plain words: an array of two-word structs
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(PF_NT) void k_filter(FilterArgs a) {
       s_excl = 0;
     } else {
       lb_store(a.flags + tile, lb_pack(epoch, LB_AGG, run));
-      const uint64_t ex = lb_lookback(a.flags, tile, 1, 0, epoch);
+      const uint64_t ex = lb_lookback(a.flags, tile, 1, 0, epoch, a.err);
       lb_store(a.flags + tile, lb_pack(epoch, LB_INC, ex + run));
       s_excl = ex;
     }
@@ -294,6 +295,7 @@ extern "C" int dakc_table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_coun
   a.flags = (uint64_t *)p;
   a.ticket = (uint32_t *)(p + fl);
   a.out = (unsigned long long *)(p + fl + 8);
+  a.err = (uint32_t *)(p + fl + 24);
   a.off_in = (const uint64_t *)(p + fl + meta);
   a.off_out = (uint64_t *)(p + fl + meta + ob);
   launch_fill_u64((uint64_t *)p, 0, (fl + meta) / 8, st);
@@ -301,12 +303,13 @@ extern "C" int dakc_table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_coun
     return DAKC_E_DEVICE;
   if (t->key_words == 1) k_filter<1><<<(uint32_t)ntiles, PF_NT, 0, st>>>(a);
   else k_filter<2><<<(uint32_t)ntiles, PF_NT, 0, st>>>(a);
-  unsigned long long out[2];
+  unsigned long long out[3];  // kept, sum of kept counts, error word
   std::vector<uint64_t> offs(nbins + 1);
-  if (hipMemcpyAsync(out, a.out, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(out, a.out, 24, hipMemcpyDeviceToHost, st) != hipSuccess ||
       (nbins && hipMemcpyAsync(offs.data(), a.off_out, ob, hipMemcpyDeviceToHost, st) != hipSuccess))
     return DAKC_E_DEVICE;
   if ((s = sync_status(st))) return s;
+  if ((s = status_of_err((uint32_t)out[2]))) return s;  // the table is unchanged on the host side
   t->n = out[0];
   t->total = out[1];
   if (nbins) std::memcpy(t->bin_offsets, offs.data(), ob);

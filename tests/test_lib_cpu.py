@@ -113,3 +113,22 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_error_bits_map_to_statuses(tmp_path):
+    # the host's mapping of a job's error word (dk_status.hpp, compiled here on
+    # its own): a look-back wait that gave up is a device error whatever else
+    # is set; then malformed input; then an overflow (the LSD recount)
+    src = tmp_path / "st.cpp"
+    src.write_text('#include <cstdio>\n#include "dk_status.hpp"\n'
+                   'int main() { for (unsigned e = 0; e < 16; ++e) std::printf("%u %d\\n", e, dk::status_of_err(e)); }\n')
+    exe = tmp_path / "st"
+    import subprocess
+    subprocess.run(["g++", "-std=c++17", f"-I{ROOT / 'dakc_amd' / 'csrc'}", f"-I{ROOT / 'include'}", str(src),
+                    "-o", str(exe)], check=True)
+    got = dict(tuple(map(int, l.split())) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                                   text=True).stdout.split("\n") if l.strip())
+    DEVICE, FORMAT, OVERFLOW = 5, 2, -100
+    for e in range(16):
+        want = DEVICE if e & 8 else FORMAT if e & 3 else OVERFLOW if e & 4 else 0
+        assert got[e] == want, e

@@ -32,7 +32,7 @@ EXPORTS = [
     "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
-    "dakc_convert_reads",
+    "dakc_convert_reads", "dakc_plan_job",
 ]
 
 
@@ -75,6 +75,16 @@ class Stats(C.Structure):
         ("path", C.c_uint32), ("msd_reruns", C.c_uint32), ("part2_ms", C.c_double),
         ("final_ms", C.c_double),
     ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("path", C.c_uint32), ("key_words", C.c_uint32), ("kmer_slots", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("b1", C.c_uint32), ("b2", C.c_uint32), ("b3", C.c_uint32),
+                ("first_level_buckets", C.c_uint32), ("l1_tiles", C.c_uint32), ("l2_groups", C.c_uint32),
+                ("final_items", C.c_uint32), ("final_threads", C.c_uint32), ("radix_passes", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class ServeStats(C.Structure):
@@ -146,6 +156,7 @@ def load() -> C.CDLL:
         "dakc_table_query": (C.c_int, [V, P(Table), V, C.c_uint64, C.c_int, V]),
         "dakc_convert_reads": (C.c_int, [V, V, C.c_uint64, C.c_int, C.c_uint32, V, C.c_uint64,
                                          P(C.c_uint64), P(C.c_uint32)]),
+        "dakc_plan_job": (C.c_int, [P(Job), C.c_int, C.c_uint32, P(PlanInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -195,3 +206,10 @@ def table_to_numpy(t: Table) -> tuple[np.ndarray, np.ndarray, np.ndarray | None]
     if t.num_bins:
         offs = np.ctypeslib.as_array(t.bin_offsets, shape=(t.num_bins + 1,)).copy()
     return keys, counts, offs
+
+
+def plan_job(job: Job, algo: int = ALGO_AUTO, num_cus: int = 0) -> dict:
+    """The geometry a job would be counted with (host only, no device call)."""
+    info = PlanInfo()
+    check(load().dakc_plan_job(C.byref(job), algo, num_cus, C.byref(info)), "dakc_plan_job")
+    return info.as_dict()

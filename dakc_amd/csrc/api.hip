@@ -354,14 +354,19 @@ int plan_choose_path(Plan *pl, int algo) {
   const uint64_t target = std::min<uint64_t>(
       cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100,
       std::max<uint64_t>(cap3 / 8, nexp / std::max(p.num_cus, 1u)));
-  uint32_t b12 = 0;
-  while ((nexp >> b12) > target) ++b12;
+  // With all P owner bins counted, the owner is the top of the first-level
+  // digit: MurmurHash64A spreads the k-mers evenly over the owners, so each
+  // owner block needs key bits only for its own 1/P of them.
   const uint32_t owners = p.owner_digit ? p.num_pes : 1;
   if (owners > 1024) return 0;
+  const uint64_t nown = (nexp + owners - 1) / owners;
+  uint32_t b12 = 0;
+  while ((nown >> b12) > target) ++b12;
+  if (b12 > bits) return 0;  // more splitting than the key has bits: few distinct keys, LSD
   uint32_t b1max = 0;
   while ((owners << (b1max + 1)) <= 1024 && b1max < 10) ++b1max;
   uint32_t b1 = std::min(std::min(b1max, (b12 + 1) / 2), bits);
-  if (b12 > b1 + 10) b1 = std::min(b1max, b12 - 10);
+  if (b12 > b1 + 10) b1 = std::min(std::min(b1max, b12 - 10), bits);
   const uint32_t b2 = std::min(std::min<uint32_t>(10, b12 > b1 ? b12 - b1 : 0), bits - b1);
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
   // L1 tile geometry: a record's slots take ceil(S / items) threads
@@ -381,7 +386,7 @@ int plan_choose_path(Plan *pl, int algo) {
   // from twice the mean at the bottom of the key range to none at the top, so
   // each quarter of an owner's top-bit range gets items sized for its own
   // densest point (2, 1.5, 1, 0.5 x the mean).
-  const double per_digit = (double)nexp / (double)(1ull << (b1 + b2f));
+  const double per_digit = (double)nown / (double)(1ull << (b1 + b2f));
   const uint32_t lc = std::min<uint32_t>(2, b1), nd2 = 1u << b2f;
   uint32_t packmax = 1;
   for (uint32_t c = 0; c < 4; ++c) {
@@ -892,9 +897,27 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   JobWs &ws = c->ws;
   uint64_t n = 0, d = 0;
   uint32_t reruns = 0;
+  bool freed = false;  // the workspace was emptied once for this path
   for (;;) {
-    if ((s = ws.reserve(pl, !job->reads_on_device))) return s;
-    if ((s = enqueue_count(ws, pl, job, (const uint8_t *)job->reads, c->num_cus, true))) return s;
+    s = ws.reserve(pl, !job->reads_on_device);
+    if (!s) s = enqueue_count(ws, pl, job, (const uint8_t *)job->reads, c->num_cus, true);
+    if (s == DAKC_E_NOMEM) {
+      // out of device memory: give back what the workspace holds from earlier
+      // counts and try again; then fall back from the MSD path to the LSD
+      // path, whose workspace is smaller
+      if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+      ws.release_job_buffers();
+      if (!freed) {
+        freed = true;
+        continue;
+      }
+      if (pl.msd) {
+        plan_choose_path(&pl, DAKC_ALGO_LSD);
+        freed = true;
+        continue;
+      }
+    }
+    if (s) return s;
     if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
     s = finish_count(ws, pl, &n, &d);
     if (s == STATUS_MSD_OVERFLOW && pl.msd) {  // skewed input: count it on the LSD path
@@ -945,6 +968,33 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
       dakc_table_free(out);
       return DAKC_E_DEVICE;
     }
+  }
+  return DAKC_OK;
+}
+
+extern "C" int dakc_plan_job(const dakc_job *job, int algo, uint32_t num_cus, dakc_plan_info *out) {
+  if (!job || !out) return DAKC_E_INVAL;
+  std::memset(out, 0, sizeof(*out));
+  if (job->format != DAKC_FMT_FIXED) return DAKC_E_INVAL;
+  Plan pl;
+  if (int s = make_plan(job, &pl)) return s;
+  pl.num_cus = num_cus ? num_cus : 256;
+  plan_choose_path(&pl, algo);
+  out->path = pl.msd ? 2 : 1;
+  out->key_words = (uint32_t)pl.kw;
+  out->kmer_slots = pl.nmax;
+  out->device_bytes = job_device_bytes(pl);
+  if (pl.msd) {
+    out->b1 = pl.b1;
+    out->b2 = pl.b2;
+    out->b3 = pl.b3;
+    out->first_level_buckets = pl.nb1;
+    out->l1_tiles = pl.ntiles1;
+    out->l2_groups = pl.ngroups;
+    out->final_items = pl.nsb;
+    out->final_threads = pl.fin_nt;
+  } else {
+    out->radix_passes = (uint32_t)pl.npasses;
   }
   return DAKC_OK;
 }

@@ -366,6 +366,16 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
   TuneEntry best;
   best.ms = 1e300;
   dakc_table t;
+  // the table entry before tuning, kept when no candidate counts the job
+  TuneEntry before;
+  bool had = false;
+  {
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    auto it = ctx->tuning.find({base.kw, size_class(base.nmax)});
+    if ((had = it != ctx->tuning.end())) before = it->second;
+  }
+  int last_err = DAKC_OK;
+  bool any = false;
   for (const Cand &cd : cands) {
     {
       std::lock_guard<std::mutex> g(ctx->tune_mu);
@@ -382,6 +392,7 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
       dakc_table_free(&t);
       if (s) {
         ok = false;
+        last_err = s;
         break;
       }
       dakc_stats st;
@@ -389,6 +400,7 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
       if (r > 0) ms.push_back(st.total_ms + (st.msd_reruns ? 1e9 : 0));
     }
     if (!ok) continue;
+    any = true;
     std::sort(ms.begin(), ms.end());
     const double med = ms[ms.size() / 2];
     if (med < best.ms * (best.ms < 1e300 ? 0.95 : 1.0)) {
@@ -399,6 +411,11 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
     }
   }
   std::lock_guard<std::mutex> g(ctx->tune_mu);
+  if (!any) {  // no candidate counted the job: the table stays as it was
+    if (had) ctx->tuning[{base.kw, size_class(base.nmax)}] = before;
+    else ctx->tuning.erase({base.kw, size_class(base.nmax)});
+    return last_err;
+  }
   ctx->tuning[{base.kw, size_class(base.nmax)}] = best;
   if (best_ms) *best_ms = best.ms;
   return DAKC_OK;

@@ -305,7 +305,7 @@ static void scheduler(ServeState *s) {
     int large_running = 0;
     const bool pipelined = s->queue.size() > 4 * s->slots.size();
     for (auto &sl : s->slots) large_running += holds_large(*sl, pipelined);
-    for (auto qi = s->queue.begin(); qi != s->queue.end();) {
+    for (auto qi = s->queue.begin(); !s->stop && qi != s->queue.end();) {
       Slot *slp = nullptr;
       for (auto &sl : s->slots)
         if (!sl->ticket) {
@@ -359,13 +359,20 @@ static void scheduler(ServeState *s) {
       any_running = true;
       progressed = true;
     }
-    if (s->stop && s->queue.empty() && !any_running && s->copying.empty()) break;
+    if (s->stop) {
+      // shutting down: jobs not admitted yet are cancelled (a job waiting
+      // for pool room that only unclaimed tables hold would otherwise wait
+      // for ever); the loop ends once the running jobs and copies are done
+      for (uint64_t id : s->queue) fail(*s, *s->tickets[id], DAKC_E_CANCELLED);
+      s->queue.clear();
+      if (!any_running && s->copying.empty()) break;
+    }
     if (!progressed) {
       if (any_running || !s->copying.empty()) {
         lk.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(50));
         lk.lock();
-      } else if (blocked) {
+      } else if (blocked && !s->stop) {
         // only unclaimed tables hold the pool: look again when a wait claims one
         s->cv_sched.wait_for(lk, std::chrono::milliseconds(1));
       } else {
@@ -455,8 +462,16 @@ extern "C" int dakc_submit(dakc_ctx *ctx, const dakc_job *job, uint64_t *ticket)
   if (st) return st;
   ServeState *s = serve_get(ctx, &st);
   if (!s) return st;
-  if (job_device_bytes(pl) > s->arena->size) return DAKC_E_REJECTED;
+  // the footprint of the path the job will take: a job whose MSD workspace
+  // can never fit the pool takes the LSD path when that one fits
   apply_tuning(ctx, &pl, ctx->opts.algo);
+  if (job_device_bytes(pl) > s->arena->size) {
+    if (!pl.msd) return DAKC_E_REJECTED;
+    Plan lsd = pl;
+    plan_choose_path(&lsd, DAKC_ALGO_LSD);
+    if (job_device_bytes(lsd) > s->arena->size) return DAKC_E_REJECTED;
+    pl = lsd;
+  }
   auto t = std::make_unique<Ticket>();
   t->job = *job;
   t->plan = pl;

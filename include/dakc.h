@@ -143,6 +143,25 @@ int dakc_count(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
 int dakc_count_device(dakc_ctx *ctx, const dakc_job *job, dakc_table *out);
 int dakc_last_stats(dakc_ctx *ctx, dakc_stats *out);
 
+/* The geometry a job would be counted with on a GPU of num_cus compute units
+ * (0 = 256), worked out on the host with no device call: for diagnostics and
+ * tuning (no counterpart in the reference, whose geometry is fixed at
+ * compile time).  algo is a DAKC_ALGO_*; the path the count takes can still
+ * change at run time (an LSD recount after an LDS overflow). */
+typedef struct dakc_plan_info {
+  uint32_t path;                 /* 1: LSD radix sort, 2: MSD partition + in-LDS count */
+  uint32_t key_words;            /* 1 (k <= 32) or 2 */
+  uint64_t kmer_slots;           /* window slots of the records: a bound on k-mers and distinct */
+  uint64_t device_bytes;         /* device workspace the count holds */
+  uint32_t b1, b2, b3;           /* MSD: key bits split by the first, second and final level */
+  uint32_t first_level_buckets;  /* MSD: owners x 2^b1 */
+  uint32_t l1_tiles, l2_groups;  /* MSD: first-level tiles, second-level groups per bucket */
+  uint32_t final_items;          /* MSD: final-level work items */
+  uint32_t final_threads;        /* MSD: threads of a final-level workgroup */
+  uint32_t radix_passes;         /* LSD: digit passes (with the owner digit) */
+} dakc_plan_info;
+int dakc_plan_job(const dakc_job *job, int algo, uint32_t num_cus, dakc_plan_info *out);
+
 /* Batched: many independent jobs, one table each (outs[i] for jobs[i]).  Jobs
  * are counted together, one multi-job launch per stage, in chunks that fit the
  * HBM budget. */

@@ -262,3 +262,30 @@ def test_served_held_device_tables_fill_pool_then_fail_cleanly():
         assert kc.serve_stats()["resident_bytes"] == 0
         t = kc.wait(kc.submit(host)[0])
         assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
+
+
+def test_served_close_cancels_jobs_that_cannot_fit():
+    """Finished tables nobody has claimed fill a small pool, so the jobs behind
+    them can only wait; closing the context cancels those jobs and returns."""
+    import threading
+    import time
+    host = _lib.generate_reads(_lib.gen_params(78, 20_000, 150))  # iid: table ~ all k-mers
+    kc = KmerCounter(k=31, read_len=150, hbm_budget=128 << 20, max_inflight=2)
+    njobs = 8
+    for _ in range(njobs):
+        kc.submit(host, table_on_device=True)
+    deadline = time.time() + 60
+    st = kc.serve_stats()
+    while time.time() < deadline:
+        st = kc.serve_stats()
+        # settled: nothing running, some tables done and unclaimed, the rest queued
+        if st["completed"] >= 1 and st["queued"] >= 1 and st["admitted"] == st["completed"] + st["failed"]:
+            time.sleep(0.2)
+            if kc.serve_stats() == st:
+                break
+        time.sleep(0.05)
+    assert st["completed"] >= 1 and st["queued"] >= 1, st
+    closer = threading.Thread(target=kc.close, daemon=True)
+    closer.start()
+    closer.join(timeout=60)
+    assert not closer.is_alive(), "dakc_ctx_destroy did not return"

@@ -9,15 +9,23 @@ forward k-mers in one u64, counts u32.  The reads are generated on the device
 and are resident in HBM before timing starts; the table stays in HBM.
 
 Prints one JSON line: k-mers counted per second (whole job), the roofline of
-the dominant kernel measured with HIP events on the stream it runs on, and a
-CPU baseline (the oracle's DAKC P-PE shape on the host cores) on a bounded
-sample of the same workload.
+the dominant kernel measured with HIP events on the stream it runs on, and
+beside them (none of it in the timed region):
+  parity       the whole table against the oracle's threaded count of the same
+               reads regenerated on the host (k-mers, distinct, checksum);
+  cpu_baseline the oracle's DAKC P-PE shape on every host core this process
+               may use, and on one, over a bounded sample of the workload;
+  batched      config 4's read sets (a bounded seeded subset, and the 128
+               smallest): one captured multi-job launch vs plain per-set counts;
+  served       Poisson-arrival requests (k in {21,31,55}): p50/p99 latency and
+               k-mers/s of continuous batching vs one job at a time.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -56,25 +64,95 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
-def cpu_baseline(w: dict, sample_reads: int, threads: int) -> dict:
+def host_cpus() -> dict:
+    """What the host offers this process: logical CPUs, the ones it may run on,
+    the thread share the environment sets (OMP_NUM_THREADS), the CPU model."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(share)) if share and share.isdigit() and int(share) > 0 else affinity
+    return {"nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": share, "model": model,
+            "threads": threads}
+
+
+def _timed_runs(fn, runs: int = 5):
+    """1 untimed warm-up, then `runs` timed; (median s, min s, last result)."""
+    fn()
+    times, res = [], None
+    for _ in range(runs):
+        res = fn()
+        times.append(res[-1])
+    return float(np.median(times)), float(min(times)), res
+
+
+def cpu_baseline(w: dict, sample_reads: int, threads: int, t1_reads: int) -> dict:
+    """The oracle's DAKC P-PE shape (T workers = T PEs over shared-memory owner
+    bins) on the host cores, over the first reads of the same workload: T =
+    every core this process may use, and T = 1.  The timer covers the count
+    phases only (file read excluded, as kcounter.cpp:262)."""
     import oracle
     from dakc_amd import _lib
 
+    can = w.get("canonical", False)
     p = _lib.gen_params(w["seed"], sample_reads, w["read_len"], genome_len=w["genome_len"],
                         sub_rate=w["sub_rate"], min_len=w.get("min_len", 0))
     reads = _lib.generate_reads(p, threads=threads)
-    can = w.get("canonical", False)
-    oracle.count_mt(reads, w["read_len"], w["k"], threads, canonical=can)  # warm-up
-    times = []
-    n = 0
-    for _ in range(3):
-        n, d, cs, el = oracle.count_mt(reads, w["read_len"], w["k"], threads, canonical=can)
-        times.append(el)
-    t = float(np.median(times))
-    return {"value": n / t, "unit": "k-mers/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample_reads} reads of the same workload ({n} k-mers), "
-                      f"oracle P-PE shape (P = threads), median of 3 after 1 warm-up, "
-                      f"{t:.3f} s each"}
+    run = lambda r, T: oracle.count_mt(r, w["read_len"], w["k"], T, canonical=can)  # noqa: E731
+    med, mn, (n, d, cs, _) = _timed_runs(lambda: run(reads, threads))
+    one = reads[:t1_reads * (w["read_len"] + 1)]
+    med1, mn1, (n1, _, _, _) = _timed_runs(lambda: run(one, 1))
+    cpus = host_cpus()
+    return {"value": n / med, "unit": "k-mers/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample_reads} reads of the same workload ({n} k-mers), oracle P-PE shape "
+                      f"(P = T = {threads} threads); median of 5 timed runs after 1 warm-up",
+            "median_s": med, "min_s": mn, "min_value": n / mn,
+            "t1": {"value": n1 / med1, "min_value": n1 / mn1, "median_s": med1, "min_s": mn1,
+                   "sample": f"first {t1_reads} reads ({n1} k-mers), one thread"},
+            "host": cpus}
+
+
+def full_parity(kc, t, w: dict, dreads: int, nbytes: int, threads: int) -> dict:
+    """Outside the timed region: the GPU's device table `t` of the whole
+    workload against the oracle's threaded count of the same reads, made again
+    on the host by the same counter-based generator (the bytes are compared
+    with the device's first).  Compared by (k-mers, distinct, order-independent
+    checksum of the (kmer, count) multiset)."""
+    import oracle
+    from dakc_amd import _lib
+    from dakc_amd.kcounter import _ptr
+    t0 = time.perf_counter()
+    kb = 8 * int(t.key_words)
+    d = int(t.n)
+    keys = kc.d2h(_ptr(t.keys), d * kb, np.uint64).reshape(d, kb // 8)
+    counts = kc.d2h(_ptr(t.counts), d * 4, np.uint32)
+    if kb == 8:
+        ascending = bool(np.all(keys[1:, 0] > keys[:-1, 0]))
+    else:
+        hi, lo = keys[:, 1], keys[:, 0]
+        ascending = bool(np.all((hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1]))))
+    gpu = {"n": int(t.total), "d": d, "checksum": oracle.checksum(keys, counts),
+           "strictly_ascending": d < 2 or ascending, "counts_ge_1": bool(d == 0 or counts.min() >= 1)}
+    del keys, counts
+    p = _lib.gen_params(w["seed"], w["num_reads"], w["read_len"], genome_len=w["genome_len"],
+                        sub_rate=w["sub_rate"], min_len=w.get("min_len", 0))
+    host = _lib.generate_reads(p, threads=threads)
+    same_input = host.size == nbytes and np.array_equal(kc.d2h(dreads, nbytes, np.uint8), host)
+    n, dd, cs, el = oracle.count_mt(host, w["read_len"], w["k"], threads, canonical=w.get("canonical", False))
+    del host
+    cpu = {"n": n, "d": dd, "checksum": cs, "count_s": el}
+    ok = (gpu["n"], gpu["d"], gpu["checksum"]) == (n, dd, cs) and gpu["strictly_ascending"] and gpu["counts_ge_1"]
+    return {"equal": bool(same_input and ok),
+            "input_bytes_equal": bool(same_input), "gpu": gpu, "cpu_oracle": cpu, "cpu_threads": threads,
+            "what": "whole workload: GPU table (n, d, checksum, ascending keys) vs the oracle's threaded "
+                    "count of the same reads regenerated on the host",
+            "seconds": time.perf_counter() - t0}
 
 
 def pre_count(kc, dreads: int, w: dict, nreads: int = 4_000_000) -> dict:
@@ -152,9 +230,38 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
-def _batch_legs(kc, sets, steps, warmup):
+def _table_checksum(kc, t) -> tuple[int, int, int]:
+    """(k-mers, distinct, order-independent checksum) of a device or host table."""
+    import oracle
+    from dakc_amd.kcounter import DeviceTable, KmerTable, _ptr
+    if isinstance(t, KmerTable):
+        return int(t.total), len(t), oracle.checksum(t.keys, t.counts)
+    raw = t.t if isinstance(t, DeviceTable) else t
+    d, kw = int(raw.n), int(raw.key_words)
+    keys = kc.d2h(_ptr(raw.keys), d * kw * 8, np.uint64).reshape(d, kw)
+    counts = kc.d2h(_ptr(raw.counts), d * 4, np.uint32)
+    return int(raw.total), d, oracle.checksum(keys, counts)
+
+
+def batch_sets(cap_reads: float, nsets: int = 256):
+    """Config 4's 256 read sets (sizes log-uniform 1e4-5e6 reads, seeded), taken
+    in seeded order while they fit `cap_reads` reads, and the 128 smallest."""
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(5e6), nsets)).astype(np.int64)
+    chosen, tot = [], 0
+    for i, nr in enumerate(sizes):
+        if tot + nr > cap_reads and chosen:
+            continue
+        chosen.append((i, int(nr)))
+        tot += int(nr)
+    small = sorted(((i, int(nr)) for i, nr in enumerate(sizes)), key=lambda x: x[1])[:nsets // 2]
+    return chosen, small
+
+
+def _batch_legs(kc, sets, steps, warmup, tune_trials=3):
     """Device-resident read sets: autotune (one set per factor of 2 in size),
-    then plain per-set counts vs one captured multi-job launch."""
+    then plain per-set counts vs one captured multi-job launch; every batched
+    table checked against its plain count by checksum."""
     from dakc_amd import _lib
     ptrs, nbs = [], []
     for i, nr in sets:
@@ -169,7 +276,7 @@ def _batch_legs(kc, sets, steps, warmup):
         classes.setdefault(int(np.log2(nb / 251)), []).append((nb, d))
     for members in classes.values():
         nb, d = sorted(members)[len(members) // 2]
-        kc.autotune(dev_ptr=d, nbytes=nb, trials=3)
+        kc.autotune(dev_ptr=d, nbytes=nb, trials=tune_trials)
     # plain: one count per set, in turn (the largest set first, untimed, so
     # the workspace is at its full size); best of two passes
     big = int(np.argmax(nbs))
@@ -177,12 +284,11 @@ def _batch_legs(kc, sets, steps, warmup):
     plain_s = 1e30
     for _ in range(2):
         t0 = time.perf_counter()
-        totals = []
         for d, nb in zip(ptrs, nbs):
-            t = kc.count_device(d, nb)
-            totals.append((int(t.total), int(t.n)))
+            kc.count_device(d, nb)
         plain_s = min(plain_s, time.perf_counter() - t0)
-    kmers = sum(x[0] for x in totals)
+    plain = [_table_checksum(kc, kc.count_device(d, nb)) for d, nb in zip(ptrs, nbs)]  # untimed
+    kmers = sum(x[0] for x in plain)
     b = kc.batch_create(ptrs, on_device=True, sizes=nbs)
     in_launch, held = b.info()
     for _ in range(warmup):
@@ -191,7 +297,7 @@ def _batch_legs(kc, sets, steps, warmup):
     dev_ms = [b.run() for _ in range(steps)]
     batch_s = (time.perf_counter() - t0) / steps
     tabs = b.tables()
-    same = all((t.total, len(t)) == x for t, x in zip(tabs, totals))
+    same = all(_table_checksum(kc, t) == x for t, x in zip(tabs, plain))
     b.close()
     for d in ptrs:
         kc.device_free(d)
@@ -200,68 +306,75 @@ def _batch_legs(kc, sets, steps, warmup):
                 same=same)
 
 
-def bench_batch(a) -> int:
-    """Many read sets of uneven size: plain per-set launches vs one captured
-    multi-job launch (autotuned), same device-resident inputs.  Headline: a
-    seeded sample of the 256 sets that fits one launch; beside it the small
-    half of the 256 sets, where per-launch costs dominate."""
+def run_batched(steps: int, warmup: int, cap_reads: float, with_small: bool = True, tune_trials: int = 3) -> dict:
     from dakc_amd import KmerCounter
-    rng = np.random.default_rng(4)
-    sizes = np.exp(rng.uniform(np.log(1e4), np.log(5e6), 256)).astype(np.int64)
-    chosen, tot = [], 0
-    for i, nr in enumerate(sizes):
-        if tot + nr > a.batch_reads_cap and chosen:
-            continue
-        chosen.append((i, int(nr)))
-        tot += int(nr)
-    small = sorted(((i, int(nr)) for i, nr in enumerate(sizes)), key=lambda x: x[1])[:128]
+    chosen, small = batch_sets(cap_reads)
     kc = KmerCounter(k=31, read_len=250)
-    print(f"[bench] batch: {len(chosen)} read sets, {tot} reads", file=sys.stderr, flush=True)
-    r = _batch_legs(kc, chosen, a.steps, a.warmup)
-    print(f"[bench] batch: small half, {len(small)} sets", file=sys.stderr, flush=True)
-    rs = _batch_legs(kc, small, a.steps, a.warmup)
+    print(f"[bench] batch: {len(chosen)} read sets, {sum(n for _, n in chosen)} reads", file=sys.stderr,
+          flush=True)
+    r = _batch_legs(kc, chosen, steps, warmup, tune_trials)
+    rs = None
+    if with_small:
+        print(f"[bench] batch: small half, {len(small)} sets", file=sys.stderr, flush=True)
+        rs = _batch_legs(kc, small, steps, warmup, tune_trials)
     import tempfile
     with tempfile.NamedTemporaryFile("r", suffix=".json") as tf:
         kc.tuning_save(tf.name)
         tuning = json.load(open(tf.name))["entries"]
     kc.close()
-    out = {"metric": "k-mers counted/s (batched)", "value": r["kmers"] / r["batch_s"], "unit": "k-mers/s",
-           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["batch_s"] * 1e3,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-           "data": "synthetic (seeded, generated on the device)",
-           "config": {"workload": f"batched: {r['sets']} of 256 read sets (sizes log-uniform "
-                                  f"1e4-5e6 reads, taken in seeded order up to {a.batch_reads_cap:g} "
-                                  f"reads), lengths 100-250 M-padded to 250, 0.5% subs, k=31",
-                      "reads": r["reads"], "kmers_per_step": r["kmers"], "parallelism": "single GPU"},
-           "plain_per_job": {"value": r["kmers"] / r["plain_s"], "ms": r["plain_s"] * 1e3},
+    out = {"workload": f"batched: {r['sets']} of the 256 read sets of config 4 (sizes log-uniform 1e4-5e6 "
+                       f"reads, taken in seeded order up to {cap_reads:g} reads), lengths 100-250 M-padded "
+                       f"to 250, 0.5% subs, k=31; device-resident reads, autotuned geometry",
+           "sets": r["sets"], "reads": r["reads"], "kmers": r["kmers"],
+           "batched_kmers_per_s": r["kmers"] / r["batch_s"], "batched_ms": r["batch_s"] * 1e3,
+           "plain_kmers_per_s": r["kmers"] / r["plain_s"], "plain_ms": r["plain_s"] * 1e3,
            "batched_over_plain": r["plain_s"] / r["batch_s"], "device_ms": r["device_ms"],
            "sets_in_captured_launch": r["in_launch"], "batch_device_bytes": r["held"],
            "tables_match_plain": r["same"],
-           "small_sets": {"workload": f"the {rs['sets']} smallest of the 256 sets ({rs['reads']} reads)",
-                          "batched_kmers_per_s": rs["kmers"] / rs["batch_s"], "batched_ms": rs["batch_s"] * 1e3,
-                          "plain_kmers_per_s": rs["kmers"] / rs["plain_s"], "plain_ms": rs["plain_s"] * 1e3,
-                          "batched_over_plain": rs["plain_s"] / rs["batch_s"],
-                          "sets_in_captured_launch": rs["in_launch"], "tables_match_plain": rs["same"]},
+           "timing": f"plain: best of 2 passes of one count per set; batched: mean of {steps} graph "
+                     f"replays after {warmup}; tables compared by checksum of every (kmer, count)",
            "tuning": tuning}
+    if rs is not None:
+        out["small_sets"] = {"workload": f"the {rs['sets']} smallest of the 256 sets ({rs['reads']} reads)",
+                             "batched_kmers_per_s": rs["kmers"] / rs["batch_s"],
+                             "batched_ms": rs["batch_s"] * 1e3,
+                             "plain_kmers_per_s": rs["kmers"] / rs["plain_s"], "plain_ms": rs["plain_s"] * 1e3,
+                             "batched_over_plain": rs["plain_s"] / rs["batch_s"],
+                             "sets_in_captured_launch": rs["in_launch"], "tables_match_plain": rs["same"]}
+    return out
+
+
+def bench_batch(a) -> int:
+    """Many read sets of uneven size: plain per-set launches vs one captured
+    multi-job launch (autotuned), same device-resident inputs."""
+    r = run_batched(a.steps, a.warmup, a.batch_reads_cap)
+    out = {"metric": "k-mers counted/s (batched)", "value": r["batched_kmers_per_s"], "unit": "k-mers/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["batched_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded, generated on the device)",
+           "config": {"workload": r["workload"], "reads": r["reads"], "kmers_per_step": r["kmers"],
+                      "parallelism": "single GPU"},
+           "batched": r}
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or host_cpus()["threads"]
         w = dict(seed=7000, read_len=250, min_len=100, genome_len=0, sub_rate=0.005, k=31)
-        out["cpu_baseline"] = cpu_baseline(w, 400_000, threads)
+        out["cpu_baseline"] = cpu_baseline(w, 400_000, threads, 100_000)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_serve(a) -> int:
+def run_served(nreq: int, slots: int, max_large: int, load: float = 0.75, check: int = 8) -> dict:
     """Poisson-arrival counting requests: continuous batching (dakc_submit /
     dakc_poll / dakc_wait, jobs overlapping on the device) vs one job at a time
-    in arrival order; latency = arrival -> table ready."""
+    in arrival order; latency = arrival -> table ready.  Every served table's
+    (k-mers, distinct) equals its count alone; `check` of them are compared by
+    checksum of every entry."""
     import threading
     from dakc_amd import KmerCounter, _lib
     rng = np.random.default_rng(5)
-    nreq = a.serve_requests
     sizes = np.exp(rng.uniform(np.log(1e4), np.log(1e7), nreq)).astype(np.int64)
     ks = rng.choice([21, 31, 55], nreq)
-    kc = KmerCounter(read_len=150, max_inflight=a.serve_slots, max_large=a.serve_max_large)
+    kc = KmerCounter(read_len=150, max_inflight=slots, max_large=max_large)
     reqs = []
     for i in range(nreq):
         p = _lib.gen_params(9000 + i, int(sizes[i]), 150, genome_len=50_000_000, sub_rate=0.005)
@@ -277,18 +390,22 @@ def bench_serve(a) -> int:
         d, nb, k = max((r for r in reqs if r[2] == kk), key=lambda r: r[1])
         kc.count_device(d, nb, k=k)
     service, totals = [], []
-    for d, nb, k in reqs:
+    checked = set(int(i) for i in np.random.default_rng(6).choice(nreq, size=min(check, nreq), replace=False))
+    sums = {}
+    for i, (d, nb, k) in enumerate(reqs):
         t0 = time.perf_counter()
         t = kc.count_device(d, nb, k=k)
         service.append(time.perf_counter() - t0)
         totals.append((int(t.total), int(t.n)))
+        if i in checked:
+            sums[i] = _table_checksum(kc, t)
     # served warm-up: the first submit allocates the pool
     for d, nb, k in reqs[:4]:
         kc.wait_device(kc.submit(None, dev_ptr=d, nbytes=nb, k=k, table_on_device=True)[0]).free()
     kmers = sum(x[0] for x in totals)
     print(f"[bench] serve: {nreq} requests, {kmers} k-mers, alone {sum(service):.3f} s",
           file=sys.stderr, flush=True)
-    lam = 0.75 / float(np.mean(service))  # 75% of one-at-a-time capacity
+    lam = load / float(np.mean(service))  # `load` of one-at-a-time capacity
     arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
     # plain: one at a time, in arrival order
     start = time.perf_counter()
@@ -305,7 +422,6 @@ def bench_serve(a) -> int:
     done_t = [None] * nreq
     tickets = [None] * nreq
     start = time.perf_counter()
-
     errors = []
 
     def submitter():
@@ -322,6 +438,7 @@ def bench_serve(a) -> int:
     th.start()
     pending = set(range(nreq))
     same = True
+    mix = {}
     while pending:
         if errors:
             raise errors[0]
@@ -331,10 +448,16 @@ def bench_serve(a) -> int:
                 pending.discard(i)
                 t = kc.wait_device(tickets[i])  # done: returns at once
                 same = same and (t.total, t.n) == totals[i]
-                t.free()
+                if i in checked:
+                    mix[i] = t  # compared after the timed leg
+                else:
+                    t.free()
         time.sleep(0.0002)
     th.join()
     served_span = max(done_t)
+    same = same and all(_table_checksum(kc, mix[i]) == sums[i] for i in checked)
+    for t in mix.values():
+        t.free()
     print(f"[bench] serve: served leg {served_span:.3f} s", file=sys.stderr, flush=True)
     lat_served = [done_t[i] - arrivals[i] for i in range(nreq)]
     # saturated: every request queued at once (closed loop), makespan
@@ -350,22 +473,30 @@ def bench_serve(a) -> int:
     for d, nb, k in reqs:
         kc.device_free(d)
     kc.close()
-    out = {"metric": "served p50/p99 latency and k-mers/s", "value": kmers / served_span,
-           "unit": "k-mers/s", "n_gpus": 1, "steps": nreq, "warmup": 4,
-           "ms_per_step": served_span * 1e3 / nreq, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded, generated on the device)",
-           "config": {"workload": f"served: {nreq} Poisson-arrival requests at 75% of one-at-a-time "
-                                  "capacity, sizes log-uniform 1e4-1e7 reads x 150 bp (genome-like, "
-                                  "0.5% subs), k uniform in {21,31,55}", "kmers": kmers,
-                      "arrival_rate_per_s": lam,
+    return {"workload": f"served: {nreq} Poisson-arrival requests at {load:.0%} of one-at-a-time capacity, "
+                        "sizes log-uniform 1e4-1e7 reads x 150 bp (genome-like, 0.5% subs), k uniform in "
+                        "{21,31,55}; device-resident reads, device tables", "requests": nreq, "kmers": kmers,
+            "arrival_rate_per_s": lam, "slots": slots, "max_large": max_large,
+            "served": {"p50_ms": _pct(lat_served, 50) * 1e3, "p99_ms": _pct(lat_served, 99) * 1e3,
+                       "kmers_per_s": kmers / served_span, "span_s": served_span},
+            "plain": {"p50_ms": _pct(lat_plain, 50) * 1e3, "p99_ms": _pct(lat_plain, 99) * 1e3,
+                      "kmers_per_s": kmers / plain_span, "span_s": plain_span},
+            "saturated": {"served_kmers_per_s": kmers / sat_served, "plain_kmers_per_s": kmers / sat_plain},
+            "tables_match_alone": same,
+            "tables_checked_by_checksum": len(checked), "serve_stats": sstats}
+
+
+def bench_serve(a) -> int:
+    r = run_served(a.serve_requests, a.serve_slots, a.serve_max_large)
+    out = {"metric": "served p50/p99 latency and k-mers/s", "value": r["served"]["kmers_per_s"],
+           "unit": "k-mers/s", "n_gpus": 1, "steps": r["requests"], "warmup": 4,
+           "ms_per_step": r["served"]["span_s"] * 1e3 / r["requests"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded, generated on the device)",
+           "config": {"workload": r["workload"], "kmers": r["kmers"],
                       "parallelism": f"single GPU, {a.serve_slots} job slots, "
                                      f"{a.serve_max_large} GPU-filling job(s) at once"},
-           "served": {"p50_ms": _pct(lat_served, 50) * 1e3, "p99_ms": _pct(lat_served, 99) * 1e3,
-                      "kmers_per_s": kmers / served_span},
-           "plain": {"p50_ms": _pct(lat_plain, 50) * 1e3, "p99_ms": _pct(lat_plain, 99) * 1e3,
-                     "kmers_per_s": kmers / plain_span},
-           "saturated": {"served_kmers_per_s": kmers / sat_served, "plain_kmers_per_s": kmers / sat_plain},
-           "tables_match_alone": same, "serve_stats": sstats}
+           "served_vs_plain": r}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -392,6 +523,14 @@ def main() -> int:
     ap.add_argument("--no-post", action="store_true",
                     help="skip the FASTQ conversion and the after-the-count histogram / query / "
                          "filter measurements")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the whole-workload check of the GPU table against the oracle's threaded count")
+    ap.add_argument("--no-batched", action="store_true", help="skip the bounded batched sub-object")
+    ap.add_argument("--no-served", action="store_true", help="skip the bounded served sub-object")
+    ap.add_argument("--line-batch-reads", type=float, default=4e6,
+                    help="plain mode's batched sub-object: read sets taken up to this many reads")
+    ap.add_argument("--line-serve-requests", type=int, default=150,
+                    help="plain mode's served sub-object: Poisson requests")
     ap.add_argument("--algo", default="auto", choices=["auto", "lsd"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also copy the read buffer device-to-device once (a plain copy kernel "
@@ -473,7 +612,12 @@ def main() -> int:
     dom_ms, dom_bytes = kernels[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     traffic = measured_traffic(a.workload, dom)
-    post = post_count(kc, t, kb) if not a.no_post else None
+    threads = a.cpu_threads or host_cpus()["threads"]
+    parity = None
+    if not a.no_parity:
+        print("[bench] parity: whole table against the oracle's threaded count", file=sys.stderr, flush=True)
+        parity = full_parity(kc, t, w, dreads, nbytes, threads)
+    post = post_count(kc, t, kb) if not a.no_post else None  # changes the table: after the parity check
     pre = pre_count(kc, dreads, w) if not a.no_post else None
     stages = {
         "path": "msd" if path == 2 else "lsd", "msd_reruns": reruns, "distinct": d,
@@ -506,14 +650,22 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
+        "parity": parity,
         "pre_count": pre,
         "post_count": post,
     }
     kc.device_free(dreads)
     kc.close()
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(w, min(a.cpu_sample_reads, w["num_reads"]), threads)
+        print(f"[bench] cpu baseline: {threads} threads and 1", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, min(a.cpu_sample_reads, w["num_reads"]), threads,
+                                           min(a.cpu_sample_reads // 2, w["num_reads"]))
+    # the metric's second half, bounded so the whole line takes a few minutes
+    if not a.no_batched:
+        out["batched"] = run_batched(max(3, min(a.steps, 5)), 1, a.line_batch_reads, with_small=True,
+                                     tune_trials=1)
+    if not a.no_served:
+        out["served"] = run_served(a.line_serve_requests, a.serve_slots, a.serve_max_large)
     print(json.dumps(out), flush=True)
     return 0
 

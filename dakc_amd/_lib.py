@@ -21,6 +21,7 @@ STATUS_NAMES = {
     9: "DAKC_E_PENDING", 10: "DAKC_E_CANCELLED",
 }
 FMT_FIXED, FMT_LINES, FMT_FASTQ = 0, 1, 2
+NOALLOC_SCRATCH = 256 << 20  # DAKC_NOALLOC_SCRATCH
 ALGO_AUTO, ALGO_LSD, ALGO_MSD = 0, 1, 2
 
 # every symbol include/dakc.h declares
@@ -32,7 +33,7 @@ EXPORTS = [
     "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
-    "dakc_convert_reads", "dakc_plan_job",
+    "dakc_convert_reads", "dakc_plan_job", "dakc_count_noalloc",
 ]
 
 
@@ -157,6 +158,7 @@ def load() -> C.CDLL:
         "dakc_convert_reads": (C.c_int, [V, V, C.c_uint64, C.c_int, C.c_uint32, V, C.c_uint64,
                                          P(C.c_uint64), P(C.c_uint32)]),
         "dakc_plan_job": (C.c_int, [P(Job), C.c_int, C.c_uint32, P(PlanInfo)]),
+        "dakc_count_noalloc": (C.c_int, [V, P(Job), V, V, C.c_uint64, V, P(C.c_uint64), P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

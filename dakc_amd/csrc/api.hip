@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -537,6 +538,49 @@ size_t msd_job_bytes(const Plan &pl, bool copy_reads) {
   return b.arena_bytes + b.meta_bytes + b.desc_bytes;
 }
 
+void msd_describe(const MsdBatch &b, size_t j, const uint8_t *dreads, uint8_t *ar, uint8_t *mt, JobDesc &D,
+                  TableLoc &L) {
+  const Plan &p = b.plans[j];
+  std::memset(&D, 0, sizeof(D));
+  D.rd = plan_reads(p, dreads);
+  D.rd.RT = p.RT1;
+  D.rd.wpr = p.wpr1;
+  const uint32_t bits = 2 * (uint32_t)p.k;
+  MsdArgs &m = D.m;
+  m.d1 = {DIG_MSD, bits - p.b1, p.b1, p.owner_digit ? p.num_pes : 1u};
+  m.d2 = {DIG_BITS, bits - p.b1 - p.b2, p.b2, 1};
+  m.d3 = {DIG_BITS, bits - p.b1 - p.b2 - p.b3, p.b3, 1};
+  m.nb1 = p.nb1;
+  m.nd2 = p.nd2;
+  m.ntiles1 = p.ntiles1;
+  m.tcap1 = p.tcap1;
+  m.G = p.G;
+  m.ngroups = p.ngroups;
+  m.cap2 = p.cap2;
+  m.fin_nt = p.fin_nt;
+  m.cls = p.cls_log | (p.b1 - p.cls_log) << 4;
+  m.packc = 0;
+  m.ipbc[0] = m.ipbc[1] = 0;
+  for (int c = 0; c < 4; ++c) {
+    m.packc |= p.packc[c] << (8 * c);
+    m.ipbc[c >> 1] |= p.ipbc[c] << (16 * (c & 1));
+  }
+  m.off1 = (uint16_t *)(ar + b.off_off1[j]);
+  m.off2 = (uint16_t *)(ar + b.off_off2[j]);
+  D.out1 = ar + b.off_out1[j];
+  D.out2 = ar + b.off_out2[j];
+  D.tcounts = (uint32_t *)(ar + b.off_counts[j]);
+  D.nd = (uint64_t *)(mt + j * META_STRIDE);
+  D.err = (uint32_t *)(mt + j * META_STRIDE + 16);
+  D.flags = (uint64_t *)(ar + b.off_flags[j]);
+  L = TableLoc();
+  L.keys = D.out1;
+  L.counts = D.tcounts;
+  L.binoff = p.owner_digit ? (uint64_t *)(ar + b.off_binoff[j]) : nullptr;
+  L.nd = D.nd;
+  L.err = D.err;
+}
+
 int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs) {
   const size_t nj = b.plans.size();
   int s;
@@ -557,45 +601,7 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
         return DAKC_E_DEVICE;
       dreads = ar + b.off_reads[j];
     }
-    JobDesc &D = desc[j];
-    std::memset(&D, 0, sizeof(D));
-    D.rd = plan_reads(p, dreads);
-    D.rd.RT = p.RT1;
-    D.rd.wpr = p.wpr1;
-    const uint32_t bits = 2 * (uint32_t)p.k;
-    MsdArgs &m = D.m;
-    m.d1 = {DIG_MSD, bits - p.b1, p.b1, p.owner_digit ? p.num_pes : 1u};
-    m.d2 = {DIG_BITS, bits - p.b1 - p.b2, p.b2, 1};
-    m.d3 = {DIG_BITS, bits - p.b1 - p.b2 - p.b3, p.b3, 1};
-    m.nb1 = p.nb1;
-    m.nd2 = p.nd2;
-    m.ntiles1 = p.ntiles1;
-    m.tcap1 = p.tcap1;
-    m.G = p.G;
-    m.ngroups = p.ngroups;
-    m.cap2 = p.cap2;
-    m.fin_nt = p.fin_nt;
-    m.cls = p.cls_log | (p.b1 - p.cls_log) << 4;
-    m.packc = 0;
-    m.ipbc[0] = m.ipbc[1] = 0;
-    for (int c = 0; c < 4; ++c) {
-      m.packc |= p.packc[c] << (8 * c);
-      m.ipbc[c >> 1] |= p.ipbc[c] << (16 * (c & 1));
-    }
-    m.off1 = (uint16_t *)(ar + b.off_off1[j]);
-    m.off2 = (uint16_t *)(ar + b.off_off2[j]);
-    D.out1 = ar + b.off_out1[j];
-    D.out2 = ar + b.off_out2[j];
-    D.tcounts = (uint32_t *)(ar + b.off_counts[j]);
-    D.nd = (uint64_t *)(mt + j * META_STRIDE);
-    D.err = (uint32_t *)(mt + j * META_STRIDE + 16);
-    D.flags = (uint64_t *)(ar + b.off_flags[j]);
-    TableLoc &L = b.locs[j];
-    L.keys = D.out1;
-    L.counts = D.tcounts;
-    L.binoff = p.owner_digit ? (uint64_t *)(ar + b.off_binoff[j]) : nullptr;
-    L.nd = D.nd;
-    L.err = D.err;
+    msd_describe(b, j, dreads, ar, mt, desc[j], b.locs[j]);
   }
   // argument block: JobDesc[nj] | pre1 | pre2 | pre3 | ticket, staged in
   // pinned memory once the previous upload from it has finished
@@ -625,9 +631,8 @@ int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs)
   return DAKC_OK;
 }
 
-static BatchArgs batch_args(const JobWs &ws, const MsdBatch &b) {
+static BatchArgs batch_args(uint8_t *dd, const MsdBatch &b, bool persist) {
   const size_t nj = b.plans.size();
-  uint8_t *dd = (uint8_t *)ws.desc.p;
   uint32_t *pre = (uint32_t *)(dd + al256(nj * sizeof(JobDesc)));
   BatchArgs a;
   a.jobs = (const JobDesc *)dd;
@@ -637,26 +642,28 @@ static BatchArgs batch_args(const JobWs &ws, const MsdBatch &b) {
   a.njobs = (uint32_t)nj;
   a.ticket = pre + 3 * (nj + 1);
   a.item0 = 0;
-  a.persist = ws.fin_persistent ? 1u : 0u;
+  a.persist = persist ? 1u : 0u;
   return a;
 }
 
 // Fill kernels and the three levels; no host transfer, so a stream capture of
-// this sequence replays it exactly.
-void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
-  hipStream_t st = ws.stream;
+// this sequence replays it exactly.  arena / meta / desc: the job's device
+// buffers (desc holds the argument block; a one-job launch reads only its
+// tickets from it); ev: 3 + MAX_PASSES stage events, or null.
+void msd_enqueue_at(hipStream_t st, const MsdBatch &b, uint8_t *arena, uint8_t *meta, uint8_t *desc,
+                    bool persist, hipEvent_t *ev) {
   const size_t nj = b.plans.size();
-  const BatchArgs a = batch_args(ws, b);
-  launch_fill_u64((uint64_t *)ws.meta.p, 0, b.meta_bytes / 8, st);
-  launch_fill_u64((uint64_t *)ws.arena.p + b.flags_off / 8, 0, b.flags_words, st);
+  const BatchArgs a = batch_args(desc, b, persist);
+  launch_fill_u64((uint64_t *)meta, 0, b.meta_bytes / 8, st);
+  launch_fill_u64((uint64_t *)arena + b.flags_off / 8, 0, b.flags_words, st);
   launch_fill_u32(a.ticket, 0, std::max<size_t>(b.fin_runs.size(), 1), st);
-  if (events) hipEventRecord(ws.ev[0], st);
+  if (ev) hipEventRecord(ev[0], st);
   const JobDesc *one = nj == 1 ? &b.descs[0] : nullptr;
   const uint32_t fin0 = b.fin_runs.empty() ? 256u : b.fin_runs[0].fin_nt;
   launch_msd_batch(b.kw, fin0, a, one, b.pre1[nj], 0, 0, st);
-  if (events) hipEventRecord(ws.ev[1], st);
+  if (ev) hipEventRecord(ev[1], st);
   launch_msd_batch(b.kw, fin0, a, one, 0, b.pre2[nj], 0, st);
-  if (events) hipEventRecord(ws.ev[2], st);
+  if (ev) hipEventRecord(ev[2], st);
   for (size_t r = 0; r < b.fin_runs.size(); ++r) {
     const MsdBatch::FinRun &fr = b.fin_runs[r];
     BatchArgs ar = a;
@@ -669,7 +676,12 @@ void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
     if (b.plans[j].owner_digit)
       launch_bin_offsets_at(b.kw, b.locs[j].keys, b.locs[j].nd, b.plans[j].num_pes, b.plans[j].seed,
                             b.locs[j].binoff, st);
-  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
+  if (ev) hipEventRecord(ev[2 + MAX_PASSES], st);
+}
+
+void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events) {
+  msd_enqueue_at(ws.stream, b, (uint8_t *)ws.arena.p, (uint8_t *)ws.meta.p, (uint8_t *)ws.desc.p,
+                 ws.fin_persistent, events ? ws.ev : nullptr);
 }
 
 int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, uint64_t *d) {
@@ -704,12 +716,26 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
       return DAKC_E_DEVICE;
     dreads = (const uint8_t *)ws.reads.p;
   }
+  lsd_enqueue(w, pl, dreads, st, num_cus, [&ws] { return ws.next_epoch(); }, events ? ws.ev : nullptr,
+              ws.loc);
+  if (hipMemcpyAsync(ws.hmeta, w.n_dev, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(ws.hmeta + 16, w.err, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+    return DAKC_E_DEVICE;
+  return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
+// The LSD path of one job on workspace w (keys_a / keys_b / counts for
+// w.key_cap keys, flags for max(tiles0, tiles_keys) tiles): histograms, one
+// sweep per digit, run-length reduce, owner-bin offsets.  next_epoch gives
+// each look-back pass a fresh flag epoch; ev: stage events or null.
+void lsd_enqueue(const Workspace &w, const Plan &pl, const uint8_t *dreads, hipStream_t st, int num_cus,
+                 const std::function<uint32_t()> &next_epoch, hipEvent_t *ev, TableLoc &loc) {
   const ReadsDesc rd = plan_reads(pl, dreads);
   launch_fill_u64(w.hist, 0, (uint64_t)pl.npasses * 256, st);
   launch_fill_u64(w.n_dev, 0, 2, st);
   launch_fill_u32(w.err, 0, 1, st);
   launch_fill_u32(w.tickets, 0, MAX_PASSES + 2, st);
-  if (events) hipEventRecord(ws.ev[0], st);
+  if (ev) hipEventRecord(ev[0], st);
   auto pass_digit = [&](int p) -> DigitSpec {
     if (pl.owner_digit && p == pl.npasses - 1) return DigitSpec{DIG_OWNER, 0, 8, pl.num_pes};
     return DigitSpec{DIG_BITS, 8u * (uint32_t)p, 8, 1};
@@ -722,11 +748,11 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
     launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
     launch_scan_digits(pl.npasses, w, st);
   }
-  if (events) hipEventRecord(ws.ev[1], st);
+  if (ev) hipEventRecord(ev[1], st);
   void *cur = w.keys_a, *other = w.keys_b;
   if (pl.nmax > 0) {
     for (int p = 0; p < pl.npasses; ++p) {
-      const uint32_t ep = ws.next_epoch();
+      const uint32_t ep = next_epoch();
       if (p == 0) {
         launch_sweep(pl.kw, true, rd, nullptr, cur, pass_digit(0), w.digit_start, w, 0, ep,
                      pl.tiles0, st);
@@ -735,22 +761,18 @@ int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t 
                      pl.tiles_keys, st);
         std::swap(cur, other);
       }
-      if (events) hipEventRecord(ws.ev[2 + p], st);
+      if (ev) hipEventRecord(ev[2 + p], st);
     }
-    const uint32_t ep = ws.next_epoch();
+    const uint32_t ep = next_epoch();
     launch_rle(pl.kw, cur, other, w, ep, pl.tiles_keys, st);
     if (pl.owner_digit) launch_bin_offsets(pl.kw, other, pl.num_pes, pl.seed, w, st);
   }
-  if (events) hipEventRecord(ws.ev[2 + MAX_PASSES], st);
-  ws.loc.keys = other;
-  ws.loc.counts = w.counts;
-  ws.loc.binoff = w.bin_offsets;
-  ws.loc.nd = w.n_dev;
-  ws.loc.err = w.err;
-  if (hipMemcpyAsync(ws.hmeta, w.n_dev, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(ws.hmeta + 16, w.err, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
-    return DAKC_E_DEVICE;
-  return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+  if (ev) hipEventRecord(ev[2 + MAX_PASSES], st);
+  loc.keys = other;
+  loc.counts = w.counts;
+  loc.binoff = w.bin_offsets;
+  loc.nd = w.n_dev;
+  loc.err = w.err;
 }
 
 // After the stream has drained: n, d and the error word, from the pinned
@@ -782,6 +804,9 @@ extern "C" int dakc_ctx_create(int device, const dakc_opts *opts, dakc_ctx **out
   }
   if (device < 0 || device >= ndev) return DAKC_E_INVAL;
   if (hipSetDevice(device) != hipSuccess) return DAKC_E_DEVICE;
+  // once per device: the allocation-free count's array (its entry point
+  // reports a failure here)
+  noalloc_setup(device);
   dakc_ctx *c = new (std::nothrow) dakc_ctx();
   if (!c) return DAKC_E_NOMEM;
   c->device = device;

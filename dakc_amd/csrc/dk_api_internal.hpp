@@ -5,6 +5,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -155,7 +156,15 @@ void msd_layout(MsdBatch &b, const std::vector<Plan> &plans, const std::vector<b
                 uint32_t solo = 0xFFFFFFFFu);
 uint32_t solo_items(const dakc_ctx *c);
 int msd_stage(JobWs &ws, MsdBatch &b, const std::vector<const dakc_job *> &jobs);
+// job j's descriptor and table location in a laid-out batch: its buffers at
+// the layout's offsets in `arena`, its status words at slot j of `meta`
+void msd_describe(const MsdBatch &b, size_t j, const uint8_t *dreads, uint8_t *arena, uint8_t *meta,
+                  JobDesc &D, TableLoc &L);
 void msd_enqueue(JobWs &ws, const MsdBatch &b, bool events);
+void msd_enqueue_at(hipStream_t st, const MsdBatch &b, uint8_t *arena, uint8_t *meta, uint8_t *desc,
+                    bool persist, hipEvent_t *ev);
+void lsd_enqueue(const Workspace &w, const Plan &pl, const uint8_t *dreads, hipStream_t st, int num_cus,
+                 const std::function<uint32_t()> &next_epoch, hipEvent_t *ev, TableLoc &loc);
 int msd_finish(const MsdBatch &b, size_t j, const void *meta_host, uint64_t *n, uint64_t *d);
 size_t job_device_bytes(const Plan &pl);
 int enqueue_count(JobWs &ws, const Plan &pl, const dakc_job *job, const uint8_t *dreads,
@@ -196,6 +205,8 @@ int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *out
 int convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int format, uint32_t read_len,
                   uint8_t *out, uint64_t cap, DevBuf *grow, uint64_t *out_nbytes, uint32_t *out_read_len);
 void serve_shutdown(dakc_ctx *c);
+// look up the allocation-free count's device array on `device` (current)
+int noalloc_setup(int device);
 
 // Owner of a device-resident result table (dakc_table.priv): the two ranges
 // of the served pool that hold its keys and its counts, kept from the job's

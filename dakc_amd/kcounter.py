@@ -145,6 +145,20 @@ class KmerCounter:
         check(self._libh.dakc_count_device(self._ctx, C.byref(job), C.byref(t)), "dakc_count_device")
         return t
 
+    def count_noalloc(self, dev_ptr: int, nbytes: int, keys_out: int, counts_out: int, cap: int,
+                      **over) -> tuple[int, int, int, np.ndarray | None]:
+        """dakc_count_noalloc: count device-resident reads into the caller's
+        device buffers (cap entries) with no allocation.  Returns (status,
+        distinct, total, bin offsets or None); the status is returned, not
+        raised, so a caller can grow its buffers on DAKC_E_NOMEM."""
+        job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        all_bins = job.num_pes > 1 and job.owner_bin < 0
+        offs = np.zeros(job.num_pes + 1, np.uint64) if all_bins else None
+        d, n = C.c_uint64(), C.c_uint64()
+        st = self._libh.dakc_count_noalloc(self._ctx, C.byref(job), keys_out or None, counts_out or None, cap,
+                                           offs.ctypes.data if all_bins else None, C.byref(d), C.byref(n))
+        return int(st), int(d.value), int(n.value), offs
+
     def last_stats(self) -> Stats:
         s = Stats()
         check(self._libh.dakc_last_stats(self._ctx, C.byref(s)), "dakc_last_stats")

@@ -162,6 +162,28 @@ typedef struct dakc_plan_info {
 } dakc_plan_info;
 int dakc_plan_job(const dakc_job *job, int algo, uint32_t num_cus, dakc_plan_info *out);
 
+/* Allocation-free count.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing, on
+ * the device or in pinned host memory: its whole workspace is one fixed
+ * device array of DAKC_NOALLOC_SCRATCH bytes that each GPU the library runs
+ * on holds until the process ends.  Every call on one GPU shares that array,
+ * so calls that overlap on one GPU (from any context) race.
+ *   - reads: device memory (job->reads_on_device = 1), DAKC_FMT_FIXED;
+ *   - the table goes to the caller's device buffers keys_out (cap * key_words
+ *     u64) and counts_out (cap u32), ascending as dakc_count_device leaves
+ *     it; bin_offsets_out (host, num_pes + 1) receives the owner-bin
+ *     boundaries when all P bins are counted (required then, else unused);
+ *   - *n_distinct / *n_total: table entries and k-mers.  When the table has
+ *     more than cap entries nothing is written, the status is DAKC_E_NOMEM
+ *     and *n_distinct says how many there are;
+ *   - a job whose workspace does not fit the array is DAKC_E_TOOBIG (with
+ *     k <= 32 about 10M k-mer window slots fit, with k <= 64 about 6M).
+ * Same counts as dakc_count_device (MSD path, LSD recount on LDS overflow). */
+#define DAKC_NOALLOC_SCRATCH (256ull << 20)
+int dakc_count_noalloc(dakc_ctx *ctx, const dakc_job *job, uint64_t *keys_out, uint32_t *counts_out,
+                       uint64_t cap, uint64_t *bin_offsets_out, uint64_t *n_distinct, uint64_t *n_total);
+
 /* Batched: many independent jobs, one table each (outs[i] for jobs[i]).  Jobs
  * are counted together, one multi-job launch per stage, in chunks that fit the
  * HBM budget. */

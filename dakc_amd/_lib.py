@@ -15,6 +15,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libdakc_amd.so"
 
 DAKC_OK = 0
+DAKC_NOMEM = 3
 STATUS_NAMES = {
     0: "DAKC_OK", 1: "DAKC_E_INVAL", 2: "DAKC_E_FORMAT", 3: "DAKC_E_NOMEM", 4: "DAKC_E_TOOBIG",
     5: "DAKC_E_DEVICE", 6: "DAKC_E_NODEVICE", 7: "DAKC_E_REJECTED", 8: "DAKC_E_NOTFOUND",
@@ -33,7 +34,7 @@ EXPORTS = [
     "dakc_memcpy_d2h", "dakc_memcpy_d2d", "dakc_batch_info", "dakc_serve_get_stats", "dakc_batch_create", "dakc_batch_run",
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
-    "dakc_convert_reads", "dakc_plan_job", "dakc_count_noalloc",
+    "dakc_convert_reads", "dakc_plan_job", "dakc_count_noalloc", "dakc_partition", "dakc_count_keys",
 ]
 
 
@@ -159,6 +160,9 @@ def load() -> C.CDLL:
                                          P(C.c_uint64), P(C.c_uint32)]),
         "dakc_plan_job": (C.c_int, [P(Job), C.c_int, C.c_uint32, P(PlanInfo)]),
         "dakc_count_noalloc": (C.c_int, [V, P(Job), V, V, C.c_uint64, V, P(C.c_uint64), P(C.c_uint64)]),
+        "dakc_partition": (C.c_int, [V, P(Job), V, C.c_uint64, V, P(C.c_uint64)]),
+        "dakc_count_keys": (C.c_int, [V, V, C.c_uint64, C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
+                                      P(Table)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

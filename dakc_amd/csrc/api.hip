@@ -328,6 +328,28 @@ int make_plan(const dakc_job *job, Plan *pl) {
   return DAKC_OK;
 }
 
+int make_plan_keys(uint64_t n, int k, uint32_t num_pes, uint64_t seed, Plan *pl) {
+  if (!pl || k < 1 || k > 64 || num_pes < 1 || num_pes > 256) return DAKC_E_INVAL;
+  if (n >= ((uint64_t)1 << 32)) return DAKC_E_TOOBIG;
+  Plan &p = *pl;
+  p = Plan();
+  p.from_keys = 1;
+  p.kw = k <= 32 ? 1 : 2;
+  p.k = k;
+  p.nmax = n;
+  p.nrec = n;  // the key count, where a record plan has its records
+  p.nbytes = n * (uint64_t)p.kw * 8;
+  p.num_pes = num_pes;
+  p.owner_bin = -1;
+  p.seed = seed ? seed : OWNER_SEED;
+  p.tile = tile_keys(p.kw);
+  p.tiles0 = p.tiles_keys = (uint32_t)((n + p.tile - 1) / p.tile);
+  p.ndig = (2 * p.k + 7) / 8;
+  p.owner_digit = num_pes > 1 ? 1 : 0;
+  p.npasses = p.ndig + p.owner_digit;
+  return DAKC_OK;
+}
+
 // MSD geometry.  A final item should hold at most tune_fill_pct (90 by
 // default) percent of what one final workgroup sorts in LDS on average.  The
 // coarsest split that meets it fixes b1 (<= 10: 1024 first-level buckets,
@@ -370,15 +392,19 @@ int plan_choose_path(Plan *pl, int algo) {
   if (b12 > b1 + 10) b1 = std::min(std::min(b1max, b12 - 10), bits);
   const uint32_t b2 = std::min(std::min<uint32_t>(10, b12 > b1 ? b12 - b1 : 0), bits - b1);
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
-  // L1 tile geometry: a record's slots take ceil(S / items) threads
-  const uint32_t wpr = (p.L + 31) / 32 + 2;
+  // L1 tile geometry: a record's slots take ceil(S / items) threads (a key
+  // array: tiles of cap1 keys)
+  const uint32_t wpr = p.from_keys ? 0 : (p.L + 31) / 32 + 2;
   const uint32_t l1_items = p.kw == 1 ? MSD_ITEMS_1 : MSD_ITEMS_2;
-  uint32_t rt = p.S ? (uint32_t)MSD_NT / ((p.S + l1_items - 1) / l1_items) : (uint32_t)cap1;
-  rt = std::min<uint32_t>(rt, L1_MAXRT);
-  rt = std::min<uint32_t>(rt, L1_RAW / p.stride);
-  rt = std::min<uint32_t>(rt, L1_CODEW / wpr);
-  rt = std::max<uint32_t>(rt, 1);
-  if ((uint64_t)rt * p.S > cap1) return 0;
+  uint32_t rt = 1;
+  if (!p.from_keys) {
+    rt = p.S ? (uint32_t)MSD_NT / ((p.S + l1_items - 1) / l1_items) : (uint32_t)cap1;
+    rt = std::min<uint32_t>(rt, L1_MAXRT);
+    rt = std::min<uint32_t>(rt, L1_RAW / p.stride);
+    rt = std::min<uint32_t>(rt, L1_CODEW / wpr);
+    rt = std::max<uint32_t>(rt, 1);
+    if ((uint64_t)rt * p.S > cap1) return 0;
+  }
   const uint32_t b2f = std::min(std::min(10u, bits - b1), b2 + 2);
   // Final items are sized to fit where keys are densest: an item over LDS
   // capacity is redone digit by digit, and its later digits' look-back slots
@@ -416,8 +442,13 @@ int plan_choose_path(Plan *pl, int algo) {
   p.nsb = owners * per_block;
   p.RT1 = rt;
   p.wpr1 = wpr;
-  p.ntiles1 = (uint32_t)((p.nrec + rt - 1) / rt);
-  p.tcap1 = rt * p.S;
+  if (p.from_keys) {
+    p.ntiles1 = (uint32_t)((p.nmax + cap1 - 1) / cap1);
+    p.tcap1 = (uint32_t)cap1;
+  } else {
+    p.ntiles1 = (uint32_t)((p.nrec + rt - 1) / rt);
+    p.tcap1 = rt * p.S;
+  }
   // canonical k-mers (min of a k-mer and its reverse complement) are twice as
   // dense as the mean at the low end of the key range: size groups for that
   const double per_tile_bucket = (double)nexp / p.ntiles1 / p.nb1 * (p.canonical ? 2.0 : 1.0);
@@ -660,7 +691,8 @@ void msd_enqueue_at(hipStream_t st, const MsdBatch &b, uint8_t *arena, uint8_t *
   if (ev) hipEventRecord(ev[0], st);
   const JobDesc *one = nj == 1 ? &b.descs[0] : nullptr;
   const uint32_t fin0 = b.fin_runs.empty() ? 256u : b.fin_runs[0].fin_nt;
-  launch_msd_batch(b.kw, fin0, a, one, b.pre1[nj], 0, 0, st);
+  if (one && b.plans[0].from_keys) launch_l1_keys(b.kw, *one, b.pre1[1], st);
+  else launch_msd_batch(b.kw, fin0, a, one, b.pre1[nj], 0, 0, st);
   if (ev) hipEventRecord(ev[1], st);
   launch_msd_batch(b.kw, fin0, a, one, 0, b.pre2[nj], 0, st);
   if (ev) hipEventRecord(ev[2], st);
@@ -745,7 +777,8 @@ void lsd_enqueue(const Workspace &w, const Plan &pl, const uint8_t *dreads, hipS
     hs.n = pl.npasses;
     for (int p = 0; p < pl.npasses; ++p) hs.d[p] = pass_digit(p);
     const int grid = (int)std::min<uint64_t>(pl.tiles0, (uint64_t)num_cus * 4);
-    launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
+    if (pl.from_keys) launch_hist_keys(pl.kw, dreads, pl.nmax, hs, pl.seed, w.hist, grid, st);
+    else launch_hist_reads(pl.kw, rd, hs, w, pl.tiles0, grid, st);
     launch_scan_digits(pl.npasses, w, st);
   }
   if (ev) hipEventRecord(ev[1], st);
@@ -753,7 +786,9 @@ void lsd_enqueue(const Workspace &w, const Plan &pl, const uint8_t *dreads, hipS
   if (pl.nmax > 0) {
     for (int p = 0; p < pl.npasses; ++p) {
       const uint32_t ep = next_epoch();
-      if (p == 0) {
+      if (p == 0 && pl.from_keys) {  // the first pass reads the key array (left unchanged)
+        launch_sweep(pl.kw, false, rd, dreads, cur, pass_digit(0), w.digit_start, w, 0, ep, pl.tiles_keys, st);
+      } else if (p == 0) {
         launch_sweep(pl.kw, true, rd, nullptr, cur, pass_digit(0), w.digit_start, w, 0, ep,
                      pl.tiles0, st);
       } else {
@@ -918,6 +953,12 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   Plan pl;
   int s = make_plan(job, &pl);
   if (s) return s;
+  return count_planned(c, pl, job, out, to_host);
+}
+
+namespace dk {
+int count_planned(dakc_ctx *c, Plan &pl, const dakc_job *job, dakc_table *out, bool to_host) {
+  int s;
   apply_tuning(c, &pl, c->opts.algo);
   JobWs &ws = c->ws;
   uint64_t n = 0, d = 0;
@@ -996,6 +1037,7 @@ static int count_common(dakc_ctx *c, const dakc_job *job, dakc_table *out, bool 
   }
   return DAKC_OK;
 }
+}  // namespace dk
 
 extern "C" int dakc_plan_job(const dakc_job *job, int algo, uint32_t num_cus, dakc_plan_info *out) {
   if (!job || !out) return DAKC_E_INVAL;

@@ -61,6 +61,8 @@ struct Plan {
   uint64_t seed = OWNER_SEED;  // owner-hash seed
   int ndig = 0, owner_digit = 0, npasses = 0;
   uint32_t canonical = 0, last_has_nl = 1;
+  // the input is an array of nmax keys (the owner side of a P-PE run), not records
+  int from_keys = 0;
   // MSD path (msd = 1): first-level bits b1 (with the owner when all bins
   // are counted), second-level b2, in-LDS b3; L1 tiles of RT1 records.  A
   // final item is `pack` consecutive second-level digits of a bucket (ipb
@@ -148,6 +150,13 @@ struct JobWs {
 };
 
 int make_plan(const dakc_job *job, Plan *pl);
+// the kernels' view of a plan's records (or keys) at device address dreads
+ReadsDesc plan_reads(const Plan &p, const uint8_t *dreads);
+// plan of counting an array of n keys of k bases (owner bins when num_pes > 1)
+int make_plan_keys(uint64_t n, int k, uint32_t num_pes, uint64_t seed, Plan *pl);
+// the count of a planned job on the context's workspace (dakc_count[_device]
+// after the plan): table to host memory or left on the device
+int count_planned(dakc_ctx *c, Plan &pl, const dakc_job *job, dakc_table *out, bool to_host);
 // multi-job MSD: lay out, stage (allocate, upload arguments, copy host reads),
 // enqueue (fills + kernels; capturable in a hipGraph), collect per-job status
 // solo: final-level items from which a job of a multi-job layout gets a

@@ -192,6 +192,12 @@ void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc
                       uint32_t n2, uint32_t n3, hipStream_t st);
 void launch_rle(int kw, const void *sorted, void *out_keys, const Workspace &ws, uint32_t epoch,
                 uint32_t grid, hipStream_t st);
+// the first MSD level over a key array (one.rd.reads: one.rd.nrec keys), one
+// workgroup per tile of msd_tile(kw) keys
+void launch_l1_keys(int kw, const JobDesc &one, uint32_t ntiles, hipStream_t st);
+// the LSD digit histograms of a key array
+void launch_hist_keys(int kw, const void *keys, uint64_t n, const HistSpecs &hs, uint64_t seed, uint64_t *hist,
+                      int grid, hipStream_t st);
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, uint64_t seed, const Workspace &ws,
                         hipStream_t st);
 void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, uint32_t num_pes, uint64_t seed,

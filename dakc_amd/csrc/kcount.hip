@@ -1075,6 +1075,76 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   }
 }
 
+// L1 from a key array (the owner side of a P-PE run counts the k-mers it
+// received, kcounter.cpp:385-411 over vectordbg): tile t is keys [t * TILE,
+// (t + 1) * TILE) of rd.reads (rd.nrec keys), ranked by the first-level digit
+// and written contiguously at tile * tcap1 with its bucket starts in off1,
+// the layout k_l1 leaves for the second level.  One tile per workgroup.
+template <typename KeyT, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_l1_keys(JobDesc one) {
+  constexpr int TILE = NT * ITEMS;
+  __shared__ KeyT stage[TILE];
+  __shared__ uint32_t s_h[1024 + 1];
+  __shared__ uint32_t s_wave[NT / 64], s_total;
+  static_assert(NT >= 1024, "one histogram slot per thread");
+  const ReadsDesc &rd = one.rd;
+  const MsdArgs &m = one.m;
+  const uint32_t tile = blockIdx.x, tid = threadIdx.x;
+  const KeyT *in = reinterpret_cast<const KeyT *>(rd.reads);
+  const uint64_t t0 = (uint64_t)tile * TILE;
+  KeyT keys[ITEMS];
+  uint32_t vmask = 0;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint64_t i = t0 + (uint64_t)j * NT + tid;
+    keys[j] = KeyT{};
+    if (i < rd.nrec) {
+      keys[j] = in[i];
+      vmask |= 1u << j;
+    }
+  }
+  s_h[tid] = 0;
+  if (tid == 0) s_h[1024] = 0;
+  __syncthreads();
+  uint32_t dr[ITEMS];  // digit << 16 | rank within the digit
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    if ((vmask >> j) & 1) {
+      const uint32_t d = digit_eval(m.d1, keys[j], rd.seed);
+      dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+    }
+  __syncthreads();
+  scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
+  const uint32_t total = s_h[m.nb1];
+  uint16_t *off = m.off1 + tile;
+  if (tid <= m.nb1) off[(uint64_t)tid * m.ntiles1] = (uint16_t)s_h[tid];
+  if (tid == 0 && m.nb1 >= NT) off[(uint64_t)NT * m.ntiles1] = (uint16_t)s_h[NT];
+  if (tid == 0 && total) atomicAdd(reinterpret_cast<unsigned long long *>(one.nd), (unsigned long long)total);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    if ((vmask >> j) & 1) stage[s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = keys[j];
+  __syncthreads();
+  write_staged<KeyT, NT, ITEMS>(stage, reinterpret_cast<KeyT *>(one.out1) + (uint64_t)tile * m.tcap1, total, tid);
+}
+
+// Digit histograms (<= 256 bins each) of a key array, for the LSD path over
+// received keys.  Grid-stride; LDS counts flushed once per workgroup.
+template <typename KeyT, int NT>
+__global__ __launch_bounds__(NT) void k_hist_keys(const KeyT *keys, uint64_t n, HistSpecs hs, uint64_t seed,
+                                                  uint64_t *hist) {
+  __shared__ uint32_t s_h[MAX_PASSES * 256];
+  const int nh = hs.n * 256;
+  for (int i = threadIdx.x; i < nh; i += NT) s_h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * (uint64_t)NT + threadIdx.x; i < n; i += (uint64_t)gridDim.x * NT) {
+    const KeyT x = keys[i];
+    for (int p = 0; p < hs.n; ++p) atomicAdd(&s_h[p * 256 + digit_eval(hs.d[p], x, seed)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nh; i += NT)
+    if (s_h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(&hist[i]), (unsigned long long)s_h[i]);
+}
+
 // L2: bucket b, group g of G consecutive L1 tiles -> its keys in second-level
 // digit order, written at (b * ngroups + g) * cap2, with the digit starts in
 // off2[b][g][digit].  Items are numbered g * nb1 + b.  Persistent like k_l1:
@@ -1718,6 +1788,19 @@ void launch_bin_offsets_at(int kw, const void *table_keys, const uint64_t *nd, u
 void launch_bin_offsets(int kw, const void *table_keys, uint32_t num_pes, uint64_t seed, const Workspace &ws,
                         hipStream_t st) {
   launch_bin_offsets_at(kw, table_keys, ws.n_dev, num_pes, seed, ws.bin_offsets, st);
+}
+
+void launch_l1_keys(int kw, const JobDesc &one, uint32_t ntiles, hipStream_t st) {
+  if (!ntiles) return;
+  if (kw == 1) k_l1_keys<uint64_t, MSD_NT, MSD_ITEMS_1><<<ntiles, MSD_NT, 0, st>>>(one);
+  else k_l1_keys<K2, MSD_NT, MSD_ITEMS_2><<<ntiles, MSD_NT, 0, st>>>(one);
+}
+
+void launch_hist_keys(int kw, const void *keys, uint64_t n, const HistSpecs &hs, uint64_t seed, uint64_t *hist,
+                      int grid, hipStream_t st) {
+  if (!n || grid <= 0) return;
+  if (kw == 1) k_hist_keys<uint64_t, 512><<<grid, 512, 0, st>>>((const uint64_t *)keys, n, hs, seed, hist);
+  else k_hist_keys<K2, 512><<<grid, 512, 0, st>>>((const K2 *)keys, n, hs, seed, hist);
 }
 
 // compute units of the stream's device (one persistent L1 workgroup each)

@@ -159,6 +159,49 @@ class KmerCounter:
                                            offs.ctypes.data if all_bins else None, C.byref(d), C.byref(n))
         return int(st), int(d.value), int(n.value), offs
 
+    def partition(self, reads=None, *, dev_ptr: int = 0, nbytes: int = 0, keys_out: int = 0, cap: int = 0,
+                  **over) -> tuple[int, np.ndarray]:
+        """Phase 1 (dakc_partition): the k-mers routed to their owner bins in
+        device memory keys_out (cap keys).  Returns (k-mers, bin offsets);
+        with keys_out = 0 only the count is returned (DAKC_E_NOMEM taken as
+        'tell me the size')."""
+        if dev_ptr:
+            job = self.job(dev_ptr, nbytes, on_device=True, **over)
+        else:
+            buf = _as_bytes(reads)
+            job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), **over)
+        nb = job.num_pes if job.num_pes > 1 and job.owner_bin < 0 else 1
+        offs = np.zeros(nb + 1, np.uint64)
+        n = C.c_uint64()
+        st = self._libh.dakc_partition(self._ctx, C.byref(job), keys_out or None, cap, offs.ctypes.data,
+                                       C.byref(n))
+        if not (st == _lib.DAKC_NOMEM and not keys_out):
+            check(st, "dakc_partition")
+        return int(n.value), offs
+
+    def count_keys(self, keys=None, *, dev_ptr: int = 0, n: int = 0, k: int | None = None, num_pes: int = 1,
+                   hash_seed: int = 0, on_device_table: bool = False):
+        """Phase 2 (dakc_count_keys): the table of a key array, host (n, key_words)
+        u64 array or device pointer with n keys.  Returns a KmerTable, or the
+        raw device Table with on_device_table."""
+        k = self.k if k is None else k
+        if dev_ptr:
+            ptr, on_dev = dev_ptr, 1
+        else:
+            arr = np.ascontiguousarray(keys, dtype=np.uint64)
+            n = arr.shape[0] if arr.ndim == 2 else arr.size // (1 if k <= 32 else 2)
+            ptr, on_dev = (arr.ctypes.data if n else 0), 0
+        t = Table()
+        check(self._libh.dakc_count_keys(self._ctx, ptr or None, n, k, num_pes, hash_seed, on_dev,
+                                         1 if on_device_table else 0, C.byref(t)), "dakc_count_keys")
+        if on_device_table:
+            return t
+        try:
+            keys_, counts, offs = _lib.table_to_numpy(t)
+            return KmerTable(keys_, counts, int(t.k), int(t.total), offs)
+        finally:
+            self._libh.dakc_table_free(C.byref(t))
+
     def last_stats(self) -> Stats:
         s = Stats()
         check(self._libh.dakc_last_stats(self._ctx, C.byref(s)), "dakc_last_stats")

@@ -162,6 +162,33 @@ typedef struct dakc_plan_info {
 } dakc_plan_info;
 int dakc_plan_job(const dakc_job *job, int algo, uint32_t num_cus, dakc_plan_info *out);
 
+/* ------------------------------------------- the two phases of a P-PE run --
+ * dakc_count does both phases of one PE's count at once.  These run them
+ * apart, as a P-PE run on several GPUs would (each GPU partitions its reads,
+ * an exchange ships bin o to PE o, each PE counts what it received). */
+
+/* Phase 1, the send side: every k-mer of the reads routed to its owner PE
+ * (kcounter.cpp:146-174, add_in_normal_packet / empty_packets, with
+ * owner_pe kcounter.cpp:50-54).  keys_out (device, cap keys of key_words
+ * u64) receives the owner bins one after another, each bin's k-mers in
+ * record order; bin_offsets_out (host) receives num_pes + 1 boundaries
+ * (all bins), or 2 when one bin is kept (owner_bin >= 0) or num_pes == 1.
+ * *n_out = k-mers routed; more than cap: DAKC_E_NOMEM, nothing written.
+ * The job's reads may be host or device memory (DAKC_FMT_FIXED). */
+int dakc_partition(dakc_ctx *ctx, const dakc_job *job, uint64_t *keys_out, uint64_t cap,
+                   uint64_t *bin_offsets_out, uint64_t *n_out);
+
+/* Phase 2, the owner side: the sorted (kmer, count) table of an array of n
+ * keys of k bases (key_words u64 each, in any order), as the owner sorts and
+ * run-length reduces the k-mers it received (kcounter.cpp:123-144 recv_kmer
+ * into vectordbg, :385-411).  With num_pes > 1 the table is split into the
+ * P owner bins (hash_seed as in dakc_job).  keys are device memory when
+ * keys_on_device (8-byte aligned), else host memory; they are not changed.
+ * table_on_device: the table stays in the context's workspace as with
+ * dakc_count_device, else it is copied to host memory as with dakc_count. */
+int dakc_count_keys(dakc_ctx *ctx, const uint64_t *keys, uint64_t n, int k, uint32_t num_pes,
+                    uint64_t hash_seed, int keys_on_device, int table_on_device, dakc_table *out);
+
 /* Allocation-free count.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  It allocates nothing, on

@@ -55,6 +55,11 @@ def load():
     lib.dko_extract_mt.restype = C.c_int64
     lib.dko_extract_mt.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32,
                                    C.c_int32, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
+    lib.dko_count_hitter.restype = C.c_int
+    lib.dko_count_hitter.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int32,
+                                     C.c_uint64, C.c_uint32, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64),
+                                     C.c_void_p, P(P(C.c_uint64)), P(P(C.c_uint32)), P(C.c_uint64), C.c_void_p,
+                                     C.c_void_p]
     lib.dko_table_checksum.restype = C.c_uint64
     lib.dko_table_checksum.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     lib.dko_free.restype = None
@@ -255,3 +260,41 @@ def convert(data, fastq: bool, read_len: int = 0):
     if st:
         raise OracleError(st)
     return out[:int(n.value)], int(L.value)
+
+
+class HitterStats(C.Structure):
+    _fields_ = [("buckets", C.c_uint64), ("normal_keys", C.c_uint64), ("heavy_pairs", C.c_uint64)]
+
+
+def count_hitter(reads, read_len, k, canonical=False, num_pes=1, owner_bin=-1, seed=0, bucket_size=10000):
+    """HITTER=1: (light (keys, counts, offsets), heavy (keys, counts, offsets), stats dict)."""
+    lib = load()
+    b = _buf(reads)
+    ptr = b.ctypes.data if b.size else None
+    lk, hk = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint64)()
+    lc, hc = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    dl, dh = C.c_uint64(), C.c_uint64()
+    nb = num_pes if num_pes > 1 and owner_bin < 0 else 1
+    lo, ho = np.zeros(nb + 1, np.uint64), np.zeros(nb + 1, np.uint64)
+    st = HitterStats()
+    r = lib.dko_count_hitter(ptr, b.size, read_len, k, int(canonical), num_pes, owner_bin, seed, bucket_size,
+                             C.byref(lk), C.byref(lc), C.byref(dl), lo.ctypes.data, C.byref(hk), C.byref(hc),
+                             C.byref(dh), ho.ctypes.data, C.byref(st))
+    if r:
+        raise OracleError(r)
+    kw = 1 if k <= 32 else 2
+
+    def take(kp, cp, d):
+        d = int(d.value)
+        try:
+            if d:
+                return (np.ctypeslib.as_array(kp, shape=(d * kw,)).copy().reshape(d, kw),
+                        np.ctypeslib.as_array(cp, shape=(d,)).copy())
+            return np.zeros((0, kw), np.uint64), np.zeros(0, np.uint32)
+        finally:
+            lib.dko_free(C.cast(kp, C.c_void_p))
+            lib.dko_free(C.cast(cp, C.c_void_p))
+
+    light = take(lk, lc, dl) + (lo,)
+    heavy = take(hk, hc, dh) + (ho,)
+    return light, heavy, {n: int(getattr(st, n)) for n, _ in HitterStats._fields_}

@@ -706,3 +706,223 @@ int dko_convert(const char *in, size_t n, int fastq, int read_len, char *out, si
     }
     return DKO_OK;
 }
+
+/* ------------------------------------------------------------ HITTER=1 --- */
+
+/* A (key, count) pair of the heavy stream, kept with its owner. */
+typedef struct {
+    u128 key;
+    uint64_t count;
+    uint32_t owner;
+} hit_pair;
+
+static int hit_pair_cmp(const void *a, const void *b) {
+    const hit_pair *x = (const hit_pair *)a, *y = (const hit_pair *)b;
+    if (x->owner != y->owner) return x->owner < y->owner ? -1 : 1;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return 0;
+}
+
+/* grow-only arrays of the two send streams */
+typedef struct {
+    uint64_t *keys, n, cap;  /* normal stream: kw words per key */
+    hit_pair *pairs;
+    uint64_t np, pcap;
+    int kw, oom;
+} hit_streams;
+
+static void hit_put_key(hit_streams *s, u128 x) {
+    if (s->n == s->cap) {
+        uint64_t nc = s->cap ? 2 * s->cap : 1024;
+        uint64_t *p = (uint64_t *)realloc(s->keys, (size_t)nc * s->kw * sizeof(uint64_t));
+        if (!p) { s->oom = 1; return; }
+        s->keys = p;
+        s->cap = nc;
+    }
+    key_put(s->keys, s->n++, s->kw, x);
+}
+
+static void hit_put_pair(hit_streams *s, u128 x, uint64_t c, uint32_t o) {
+    if (s->np == s->pcap) {
+        uint64_t nc = s->pcap ? 2 * s->pcap : 256;
+        hit_pair *p = (hit_pair *)realloc(s->pairs, (size_t)nc * sizeof(hit_pair));
+        if (!p) { s->oom = 1; return; }
+        s->pairs = p;
+        s->pcap = nc;
+    }
+    s->pairs[s->np].key = x;
+    s->pairs[s->np].count = c;
+    s->pairs[s->np].owner = o;
+    s->np++;
+}
+
+/* kcounter.cpp:212-245 (flush_buffer with HITTER) and :176-210
+ * (send2sendbuf): the bucket's k-mers sorted and run-length reduced; a k-mer
+ * seen once or twice goes to its owner as that many plain k-mers, one seen
+ * three or more times as one (k-mer, count) pair.  An empty bucket sends
+ * nothing (the reference would send kcount_buffer[0] with count 1). */
+static void hit_flush(uint64_t *buf, uint64_t *tmp, uint64_t nb, int k, int kw, uint32_t num_pes,
+                      int32_t owner_bin, uint64_t seed, hit_streams *s, dko_hitter_stats *st) {
+    if (nb == 0) return;
+    st->buckets++;
+    sort_keys(buf, tmp, nb, k);
+    uint64_t run = 1;
+    for (uint64_t i = 1; i <= nb; i++) {
+        if (i < nb && key_at(buf, i, kw) == key_at(buf, i - 1, kw)) {
+            run++;
+            continue;
+        }
+        const u128 x = key_at(buf, i - 1, kw);
+        const uint32_t o = num_pes > 1 ? owner_of(x, kw, num_pes, seed) : 0;
+        if (num_pes <= 1 || owner_bin < 0 || o == (uint32_t)owner_bin) {
+            if (run <= 2) {
+                for (uint64_t c = 0; c < run; c++) hit_put_key(s, x);
+                st->normal_keys += run;
+            } else {
+                hit_put_pair(s, x, run, o);
+                st->heavy_pairs++;
+            }
+        }
+        run = 1;
+    }
+}
+
+int dko_count_hitter(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                     int32_t owner_bin, uint64_t seed, uint32_t bucket_size, uint64_t **light_keys,
+                     uint32_t **light_counts, uint64_t *d_light, uint64_t *light_offsets, uint64_t **heavy_keys,
+                     uint32_t **heavy_counts, uint64_t *d_heavy, uint64_t *heavy_offsets, dko_hitter_stats *st) {
+    if (!light_keys || !light_counts || !d_light || !heavy_keys || !heavy_counts || !d_heavy || !st)
+        return DKO_E_INVAL;
+    *light_keys = *heavy_keys = NULL;
+    *light_counts = *heavy_counts = NULL;
+    *d_light = *d_heavy = 0;
+    memset(st, 0, sizeof(*st));
+    if (k < 1 || k > 64 || num_pes < 1 || read_len < 1) return DKO_E_INVAL;
+    if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return DKO_E_INVAL;
+    /* the while condition of read_till_buf_max compares with bucket_size -
+     * READLEN: below 0 it would never stop filling the buffer */
+    if (bucket_size <= (uint32_t)read_len) return DKO_E_INVAL;
+    seed = seed_or_default(seed);
+    int64_t nrec = count_records(reads, nbytes, read_len);
+    if (nrec < 0) return (int)-nrec;
+    const int kw = key_words_for(k);
+    const size_t stride = (size_t)read_len + 1;
+    const uint64_t per_read = read_len >= k ? (uint64_t)(read_len - k + 1) : 0;
+    const uint64_t bcap = bucket_size + per_read + 1;
+    uint64_t *buf = (uint64_t *)malloc((size_t)bcap * kw * sizeof(uint64_t));
+    uint64_t *tmp = (uint64_t *)malloc((size_t)bcap * kw * sizeof(uint64_t));
+    uint8_t *codes = (uint8_t *)malloc((size_t)read_len + 1);
+    hit_streams s;
+    memset(&s, 0, sizeof(s));
+    s.kw = kw;
+    if (!buf || !tmp || !codes) {
+        free(buf); free(tmp); free(codes);
+        return DKO_E_NOMEM;
+    }
+    /* kcounter.cpp:247-305 with read_till_buf_max (kcounter_funcs.cpp:69-111):
+     * records are parsed into the bucket while it holds at most
+     * bucket_size - READLEN k-mers; then the bucket is flushed */
+    uint64_t r = 0;
+    while (r < (uint64_t)nrec) {
+        emit_ctx c = {k, canonical, kw, 1, -1, seed, buf, bcap, 0};
+        while (c.n <= (uint64_t)(bucket_size - (uint32_t)read_len) && r < (uint64_t)nrec) {
+            parse_record(&c, reads + r * stride, read_len, codes);
+            r++;
+        }
+        hit_flush(buf, tmp, c.n, k, kw, num_pes, owner_bin, seed, &s, st);
+    }
+    free(buf); free(tmp); free(codes);
+    if (s.oom) {
+        free(s.keys); free(s.pairs);
+        return DKO_E_NOMEM;
+    }
+    /* the owner side, kcounter.cpp:325-384, per owner in owner order:
+     * heavy pairs sorted and merged (sort_and_merge_duplicate_kmer_packets,
+     * :56-104); the plain k-mers sorted and run-length reduced; a run whose
+     * k-mer is heavy adds to the heavy count (binary_search, :60-73),
+     * every other run is a light entry */
+    const int all = num_pes > 1 && owner_bin < 0;
+    const uint32_t P = all ? num_pes : 1;
+    qsort(s.pairs, (size_t)s.np, sizeof(hit_pair), hit_pair_cmp);
+    uint64_t hd = 0;  /* merged heavy pairs, in place */
+    for (uint64_t i = 0; i < s.np; i++) {
+        if (hd && s.pairs[hd - 1].key == s.pairs[i].key && s.pairs[hd - 1].owner == s.pairs[i].owner)
+            s.pairs[hd - 1].count += s.pairs[i].count;
+        else
+            s.pairs[hd++] = s.pairs[i];
+    }
+    /* plain k-mers grouped by owner, each group sorted */
+    uint64_t *own = (uint64_t *)calloc((size_t)P + 1, sizeof(uint64_t));
+    uint64_t *grp = (uint64_t *)malloc(((size_t)s.n + 1) * kw * sizeof(uint64_t));
+    uint64_t *gtmp = (uint64_t *)malloc(((size_t)s.n + 1) * kw * sizeof(uint64_t));
+    uint64_t *lk = (uint64_t *)malloc(((size_t)s.n + 1) * kw * sizeof(uint64_t));
+    uint32_t *lc = (uint32_t *)malloc(((size_t)s.n + 1) * sizeof(uint32_t));
+    uint64_t *hk = (uint64_t *)malloc(((size_t)hd + 1) * kw * sizeof(uint64_t));
+    uint32_t *hc = (uint32_t *)malloc(((size_t)hd + 1) * sizeof(uint32_t));
+    if (!own || !grp || !gtmp || !lk || !lc || !hk || !hc) {
+        free(own); free(grp); free(gtmp); free(lk); free(lc); free(hk); free(hc); free(s.keys); free(s.pairs);
+        return DKO_E_NOMEM;
+    }
+    for (uint64_t i = 0; i < s.n; i++)
+        own[(all ? owner_of(key_at(s.keys, i, kw), kw, num_pes, seed) : 0) + 1]++;
+    for (uint32_t o = 0; o < P; o++) own[o + 1] += own[o];
+    {
+        uint64_t *pos = (uint64_t *)malloc((size_t)P * sizeof(uint64_t));
+        if (!pos) {
+            free(own); free(grp); free(gtmp); free(lk); free(lc); free(hk); free(hc); free(s.keys); free(s.pairs);
+            return DKO_E_NOMEM;
+        }
+        memcpy(pos, own, (size_t)P * sizeof(uint64_t));
+        for (uint64_t i = 0; i < s.n; i++) {
+            const u128 x = key_at(s.keys, i, kw);
+            key_put(grp, pos[all ? owner_of(x, kw, num_pes, seed) : 0]++, kw, x);
+        }
+        free(pos);
+    }
+    free(s.keys);
+    uint64_t dl = 0, h0 = 0;
+    for (uint32_t o = 0; o < P; o++) {
+        uint64_t h1 = h0;
+        while (h1 < hd && (!all || s.pairs[h1].owner == o)) h1++;
+        if (light_offsets) light_offsets[o] = dl;
+        if (heavy_offsets) heavy_offsets[o] = h0;
+        const uint64_t b0 = own[o], b1 = own[o + 1];
+        sort_keys(grp + b0 * kw, gtmp, b1 - b0, k);
+        uint64_t run = 1;
+        for (uint64_t i = b0 + 1; i <= b1 && b1 > b0; i++) {
+            if (i < b1 && key_at(grp, i, kw) == key_at(grp, i - 1, kw)) {
+                run++;
+                continue;
+            }
+            const u128 x = key_at(grp, i - 1, kw);
+            uint64_t lo = h0, hi = h1; /* lower bound of x among this owner's heavy pairs */
+            while (lo < hi) {
+                const uint64_t mid = lo + (hi - lo) / 2;
+                if (s.pairs[mid].key < x) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < h1 && s.pairs[lo].key == x) {
+                s.pairs[lo].count += run;
+            } else {
+                key_put(lk, dl, kw, x);
+                lc[dl++] = run > 0xFFFFFFFFULL ? 0xFFFFFFFFu : (uint32_t)run;
+            }
+            run = 1;
+        }
+        h0 = h1;
+    }
+    if (light_offsets) light_offsets[P] = dl;
+    if (heavy_offsets) heavy_offsets[P] = hd;
+    for (uint64_t i = 0; i < hd; i++) {
+        key_put(hk, i, kw, s.pairs[i].key);
+        hc[i] = s.pairs[i].count > 0xFFFFFFFFULL ? 0xFFFFFFFFu : (uint32_t)s.pairs[i].count;
+    }
+    free(own); free(grp); free(gtmp); free(s.pairs);
+    *light_keys = lk;
+    *light_counts = lc;
+    *d_light = dl;
+    *heavy_keys = hk;
+    *heavy_counts = hc;
+    *d_heavy = hd;
+    return DKO_OK;
+}

@@ -100,6 +100,29 @@ uint64_t dko_table_checksum(const uint64_t *keys, const uint32_t *counts, uint64
 
 void dko_free(void *p);
 
+/* HITTER=1 (kcounter.cpp:56-104, 176-245, 325-384): records are taken into a
+ * bucket while it holds at most bucket_size - read_len k-mers
+ * (read_till_buf_max, kcounter_funcs.cpp:69-111; KCOUNT_BUCKET_SIZE 10000 in
+ * dakc/Makefile:11); each bucket is sorted and run-length reduced, a k-mer
+ * seen once or twice in it is sent to its owner as that many plain k-mers, one
+ * seen three or more times as one (k-mer, count) pair.  Each owner merges its
+ * heavy pairs into the heavy table, counts its plain k-mers, adds the runs of
+ * heavy k-mers to the heavy table and keeps the others as the light table.
+ * Both tables are ascending per owner bin and disjoint; their union is the
+ * HITTER=0 table.  With num_pes > 1 and owner_bin == -1 both are the
+ * concatenation of the P owners' tables (offsets: num_pes + 1 entries each,
+ * may be NULL); with owner_bin >= 0 only that owner's.  Tables are malloc'd
+ * (dko_free).  bucket_size must exceed read_len. */
+typedef struct {
+    uint64_t buckets;      /* buckets flushed (with at least one k-mer) */
+    uint64_t normal_keys;  /* plain k-mers sent */
+    uint64_t heavy_pairs;  /* (k-mer, count) pairs sent */
+} dko_hitter_stats;
+int dko_count_hitter(const char *reads, size_t nbytes, int read_len, int k, int canonical, uint32_t num_pes,
+                     int32_t owner_bin, uint64_t seed, uint32_t bucket_size, uint64_t **light_keys,
+                     uint32_t **light_counts, uint64_t *d_light, uint64_t *light_offsets, uint64_t **heavy_keys,
+                     uint32_t **heavy_counts, uint64_t *d_heavy, uint64_t *heavy_offsets, dko_hitter_stats *st);
+
 /* Before the count: FASTQ / line input to fixed-width records (syngen/fq2txtmaker.sh:12-18). */
 int dko_convert(const char *in, size_t n, int fastq, int read_len, char *out, size_t cap,
                 uint64_t *out_n, uint32_t *width);

@@ -128,3 +128,43 @@ def count(buf: bytes, read_len: int, k: int, canonical: bool = False,
 def revcomp_value(v: int, k: int) -> int:
     s = "".join("CATG"[(v >> (2 * (k - 1 - j))) & 3] for j in range(k))
     return value(s[::-1].translate(COMP))
+
+
+def hitter(buf: bytes, read_len: int, k: int, bucket_size: int, canonical: bool = False, num_pes: int = 1,
+           owner_bin: int = -1, seed: int = OWNER_SEED):
+    """HITTER=1 as sets: buckets of whole records (a record joins the bucket
+    while the bucket holds at most bucket_size - read_len k-mers); a k-mer is
+    heavy when it is seen three or more times in some bucket; light and heavy
+    entries carry the k-mer's total count.  Returns ({key: count} light,
+    {key: count} heavy, stats) for the owners kept (all, or owner_bin)."""
+    recs = list(records(buf, read_len))
+    per = [kmers(r + b"\n", read_len, k, canonical) for r in recs]
+    buckets, cur, n = [], [], 0
+    for km in per:
+        if n > bucket_size - read_len:
+            buckets.append(cur)
+            cur, n = [], 0
+        cur.extend(km)
+        n += len(km)
+    if recs:
+        buckets.append(cur)
+    keep = (lambda v: True) if num_pes <= 1 or owner_bin < 0 else (
+        lambda v: owner(v, k, num_pes, seed) == owner_bin)
+    heavy_set, total = set(), Counter()
+    stats = {"buckets": 0, "normal_keys": 0, "heavy_pairs": 0}
+    for bk in buckets:
+        if not bk:
+            continue
+        stats["buckets"] += 1
+        for v, c in Counter(bk).items():
+            if not keep(v):
+                continue
+            total[v] += c
+            if c >= 3:
+                heavy_set.add(v)
+                stats["heavy_pairs"] += 1
+            else:
+                stats["normal_keys"] += c
+    light = {v: c for v, c in total.items() if v not in heavy_set}
+    heavy = {v: c for v, c in total.items() if v in heavy_set}
+    return light, heavy, stats

@@ -35,6 +35,7 @@ EXPORTS = [
     "dakc_batch_tables", "dakc_batch_destroy", "dakc_autotune", "dakc_tuning_save",
     "dakc_tuning_load", "dakc_table_histogram", "dakc_table_filter", "dakc_table_query",
     "dakc_convert_reads", "dakc_plan_job", "dakc_count_noalloc", "dakc_partition", "dakc_count_keys",
+    "dakc_count_hitter",
 ]
 
 
@@ -87,6 +88,14 @@ class PlanInfo(C.Structure):
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class HitterStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("buckets", "normal_keys", "heavy_pairs", "kmers", "bytes_sent",
+                                          "bytes_without")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
 class ServeStats(C.Structure):
@@ -163,6 +172,7 @@ def load() -> C.CDLL:
         "dakc_partition": (C.c_int, [V, P(Job), V, C.c_uint64, V, P(C.c_uint64)]),
         "dakc_count_keys": (C.c_int, [V, V, C.c_uint64, C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
                                       P(Table)]),
+        "dakc_count_hitter": (C.c_int, [V, P(Job), C.c_uint32, P(Table), P(Table), P(HitterStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -23,7 +23,7 @@ LIB = PKG / "libdakc_amd.so"
 CLI = PKG / "dakc"
 ORACLE_LIB = ROOT / "oracle" / "_build" / "libdakc_oracle.so"
 
-HIP_SOURCES = ["kcount.hip", "api.hip", "serve.hip", "batch.hip", "post.hip", "input.hip", "noalloc.hip", "phases.hip"]
+HIP_SOURCES = ["kcount.hip", "api.hip", "serve.hip", "batch.hip", "post.hip", "input.hip", "noalloc.hip", "phases.hip", "hitter.hip"]
 ARCH = os.environ.get("DAKC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

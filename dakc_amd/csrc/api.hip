@@ -873,6 +873,7 @@ extern "C" int dakc_ctx_destroy(dakc_ctx *c) {
   c->ws.~JobWs();
   new (&c->ws) JobWs();
   for (DevBuf *b : {&c->post_a, &c->post_b, &c->conv_in, &c->conv_nl, &c->conv_out}) b->release();
+  c->hit.release();
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return DAKC_OK;

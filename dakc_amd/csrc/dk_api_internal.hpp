@@ -194,6 +194,20 @@ int size_class(uint64_t nmax);
 void apply_tuning(const dakc_ctx *c, Plan *pl, int algo);
 }  // namespace dk
 
+namespace dk {
+// device buffers of the HITTER count (grow-only, held by the context): the
+// bucket chain, the k-mers in record order, the send streams, and the light
+// and heavy tables of the last count
+struct HitterBufs {
+  DevBuf reads, meta, c, tiles, P, J, starts, keys, norm, pkeys, pcounts, lk, lc, hk, hc, hoff;
+  void release() {
+    for (DevBuf *b : {&reads, &meta, &c, &tiles, &P, &J, &starts, &keys, &norm, &pkeys, &pcounts, &lk, &lc, &hk,
+                      &hc, &hoff})
+      b->release();
+  }
+};
+}  // namespace dk
+
 struct dakc_ctx {
   int device = 0;
   int num_cus = 256;
@@ -207,6 +221,7 @@ struct dakc_ctx {
   std::map<std::pair<int, int>, dk::TuneEntry> tuning;  // (kw, size class) -> parameters
   dk::DevBuf post_a, post_b;  // scratch of the table histogram / filter / query
   dk::DevBuf conv_in, conv_nl, conv_out;  // FASTQ / line input: staged bytes, newlines, records
+  dk::HitterBufs hit;
 };
 
 namespace dk {
@@ -216,6 +231,8 @@ int convert_reads(dakc_ctx *c, const uint8_t *in, uint64_t nbytes, int format, u
 void serve_shutdown(dakc_ctx *c);
 // look up the allocation-free count's device array on `device` (current)
 int noalloc_setup(int device);
+// stable in-place compaction of a device table to its entries with count >= min_count
+int table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_count);
 
 // Owner of a device-resident result table (dakc_table.priv): the two ranges
 // of the served pool that hold its keys and its counts, kept from the job's

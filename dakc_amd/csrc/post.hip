@@ -278,6 +278,14 @@ extern "C" int dakc_table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_coun
   int s = post_check(ctx, t);
   if (s) return s;
   if (t->n == 0 || min_count <= 1) return DAKC_OK;  // every entry has count >= 1
+  return dk::table_filter(ctx, t, min_count);
+}
+
+// Stable in-place compaction of a device table to its entries with count >=
+// min_count (with min_count = 1: drop the entries whose count was zeroed).
+int dk::table_filter(dakc_ctx *ctx, dakc_table *t, uint32_t min_count) {
+  int s;
+  if (t->n == 0) return DAKC_OK;
   hipStream_t st = ctx->stream;
   const uint64_t ntiles = (t->n + PF_TILE - 1) / PF_TILE;
   if (ntiles > 0xFFFFFFFFull) return DAKC_E_TOOBIG;

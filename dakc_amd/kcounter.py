@@ -202,6 +202,31 @@ class KmerCounter:
         finally:
             self._libh.dakc_table_free(C.byref(t))
 
+    def count_hitter(self, reads=None, *, dev_ptr: int = 0, nbytes: int = 0, bucket_size: int = 0,
+                     table_on_device: bool = False, **over):
+        """HITTER=1 (dakc_count_hitter): (light, heavy, stats).  Host tables are
+        KmerTables; with table_on_device the raw device Tables (valid until the
+        next HITTER count on this context)."""
+        if dev_ptr:
+            job = self.job(dev_ptr, nbytes, on_device=True, table_on_device=table_on_device, **over)
+        else:
+            buf = _as_bytes(reads)
+            job = self.job(buf.ctypes.data if buf.size else 0, int(buf.size), table_on_device=table_on_device,
+                           **over)
+        lt, ht, st = Table(), Table(), _lib.HitterStats()
+        check(self._libh.dakc_count_hitter(self._ctx, C.byref(job), bucket_size, C.byref(lt), C.byref(ht),
+                                           C.byref(st)), "dakc_count_hitter")
+        if table_on_device:
+            return lt, ht, st.as_dict()
+        out = []
+        for t in (lt, ht):
+            try:
+                keys, counts, offs = _lib.table_to_numpy(t)
+                out.append(KmerTable(keys, counts, int(t.k), int(t.total), offs))
+            finally:
+                self._libh.dakc_table_free(C.byref(t))
+        return out[0], out[1], st.as_dict()
+
     def last_stats(self) -> Stats:
         s = Stats()
         check(self._libh.dakc_last_stats(self._ctx, C.byref(s)), "dakc_last_stats")

@@ -189,6 +189,30 @@ int dakc_partition(dakc_ctx *ctx, const dakc_job *job, uint64_t *keys_out, uint6
 int dakc_count_keys(dakc_ctx *ctx, const uint64_t *keys, uint64_t n, int k, uint32_t num_pes,
                     uint64_t hash_seed, int keys_on_device, int table_on_device, dakc_table *out);
 
+/* HITTER=1 (kcounter.cpp:56-104, 176-245, 325-384; off in the shipped build,
+ * dakc/Makefile:11): the send side pre-aggregates.  Records go into a bucket
+ * while it holds at most bucket_size - read_len k-mers (read_till_buf_max,
+ * kcounter_funcs.cpp:69-111; 0 = the reference's KCOUNT_BUCKET_SIZE, 10000;
+ * read_len < bucket_size <= 10240).  A k-mer seen once or twice in its
+ * bucket is sent as that many plain k-mers, one seen three or more times as
+ * one (k-mer, count) pair.  The owner side gives two tables, both ascending
+ * per owner bin and disjoint: heavy (the k-mers sent as a pair from some
+ * bucket, with their total count) and light (all others); their union is
+ * dakc_count's table.  The job's table_on_device chooses device tables in the
+ * context's workspace (valid until the next HITTER count) or host copies.
+ * stats (may be NULL) counts what the send side ships. */
+typedef struct dakc_hitter_stats {
+  uint64_t buckets;        /* buckets sent (with at least one k-mer) */
+  uint64_t normal_keys;    /* plain k-mers sent */
+  uint64_t heavy_pairs;    /* (k-mer, count) pairs sent */
+  uint64_t kmers;          /* k-mers of the owners counted (plain + the pairs' counts) */
+  uint64_t bytes_sent;     /* normal_keys * key bytes + heavy_pairs * (key bytes + 8: a u64 count,
+                            * as the reference's heavy packet, kcounter.cpp:176-188) */
+  uint64_t bytes_without;  /* kmers * key bytes: what the same owners receive with HITTER=0 */
+} dakc_hitter_stats;
+int dakc_count_hitter(dakc_ctx *ctx, const dakc_job *job, uint32_t bucket_size, dakc_table *light,
+                      dakc_table *heavy, dakc_hitter_stats *stats);
+
 /* Allocation-free count.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  It allocates nothing, on

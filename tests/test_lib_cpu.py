@@ -95,7 +95,8 @@ def test_ctypes_structs_match_the_header(tmp_path):
     # the ctypes layouts of the binding are the C layouts of include/dakc.h
     structs = {"dakc_job": _lib.Job, "dakc_table": _lib.Table, "dakc_opts": _lib.Opts,
                "dakc_stats": _lib.Stats, "dakc_serve_stats": _lib.ServeStats,
-               "dakc_gen_params": _lib.GenParams, "dakc_plan_info": _lib.PlanInfo}
+               "dakc_gen_params": _lib.GenParams, "dakc_plan_info": _lib.PlanInfo,
+               "dakc_hitter_stats": _lib.HitterStats}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dakc.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

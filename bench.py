@@ -501,12 +501,165 @@ def bench_serve(a) -> int:
     return 0
 
 
+# config 3: one PE's share of a 64-PE run -- a 2 Gbp genome, 250 bp reads at
+# 20x (1.6e8 reads) with 1% substitutions and 0.1% of bases in N runs (runs of
+# 1-10 bases: start probability 0.001 / 5.5), k = 63, P = 64
+PE64 = dict(seed=3033, num_reads=160_000_000, read_len=250, genome_len=2_000_000_000, sub_rate=0.01,
+            n_rate=0.001 / 5.5, n_run_max=10, k=63, num_pes=64)
+
+
+def _pe64_params(first: int, count: int):
+    from dakc_amd import _lib
+    w = PE64
+    return _lib.gen_params(w["seed"], count, w["read_len"], genome_len=w["genome_len"], sub_rate=w["sub_rate"],
+                           n_rate=w["n_rate"], n_run_max=w["n_run_max"], first_read=first)
+
+
+def hitter_leg(kc, dreads: int, nbytes: int, table_checksum=None, **job) -> dict:
+    """HITTER=1 on device reads (once, after one untimed call): what leaves the
+    send side with and without the per-bucket pre-aggregation; light + heavy
+    checked against the plain table (sizes, and the additive checksum of every
+    entry) when its checksum is given."""
+    kc.count_hitter(dev_ptr=dreads, nbytes=nbytes, table_on_device=True, **job)
+    t0 = time.perf_counter()
+    lt, ht, st = kc.count_hitter(dev_ptr=dreads, nbytes=nbytes, table_on_device=True, **job)
+    el = time.perf_counter() - t0
+    out = {"bucket_size": 10000, "stats": st, "ms": el * 1e3, "light": int(lt.n), "heavy": int(ht.n),
+           "bytes_ratio": st["bytes_sent"] / max(st["bytes_without"], 1),
+           "timing": "wall clock around one synchronous call (bucket chain, record-order k-mers, per-bucket "
+                     "counts, both owner-side counts and the merge), not in the timed counts"}
+    if table_checksum is not None:
+        n, d, cs = table_checksum
+        _, dl, cl = _table_checksum(kc, lt)
+        _, dh, ch = _table_checksum(kc, ht)
+        out["union_is_plain_table"] = bool(dl + dh == d and (cl + ch) % 2**64 == cs
+                                           and int(lt.total) + int(ht.total) == n)
+    from dakc_amd import _lib
+    for t in (lt, ht):
+        kc._libh.dakc_table_free(_lib.C.byref(t))
+    return out
+
+
+def bench_pe64(a) -> int:
+    """Config 3 at its real shape on one GPU: (i) PE 0's Phase 1, its 1/64 of
+    the reads partitioned into the 64 owner bins; (ii) PE 0's Phase 2, owner
+    bin 0 of all 1.6e8 reads (every sender's k-mers for PE 0, selected from
+    device-generated chunks) counted; (iii) what HITTER saves PE 0's send side.
+    Parity: two bins of (i) against the oracle's extraction, element for
+    element; (ii)'s table against the oracle's streamed bin-0 count by checksum."""
+    import oracle
+    from dakc_amd import KmerCounter, _lib
+    w = PE64
+    L, k, P = w["read_len"], w["k"], w["num_pes"]
+    threads = a.cpu_threads or host_cpus()["threads"]
+    kc = KmerCounter(k=k, read_len=L, num_pes=P, owner_bin=-1)
+    kb = 16
+    # (i) PE 0's block of records (fqreader.cpp:43-52 gives PE p the p-th 1/64)
+    nr0 = w["num_reads"] // P
+    nb0 = nr0 * (L + 1)
+    d0 = kc.device_alloc(nb0)
+    kc.generate_device(_pe64_params(0, nr0), d0, nb0)
+    n0, _ = kc.partition(dev_ptr=d0, nbytes=nb0)
+    dk = kc.device_alloc(n0 * kb)
+    kc.partition(dev_ptr=d0, nbytes=nb0, keys_out=dk, cap=n0)  # warm-up
+    times = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        n0b, offs = kc.partition(dev_ptr=d0, nbytes=nb0, keys_out=dk, cap=n0)
+        times.append(time.perf_counter() - t0)
+    part_s = float(np.median(times))
+    host0 = _lib.generate_reads(_pe64_params(0, nr0), threads=threads)
+    same_in = np.array_equal(kc.d2h(d0, nb0, np.uint8), host0)
+    ok = same_in and n0b == n0
+    for b in (0, 37):
+        got = kc.d2h(dk + int(offs[b]) * kb, int(offs[b + 1] - offs[b]) * kb, np.uint64).reshape(-1, 2)
+        want = oracle.extract_mt(host0, L, k, threads, False, P, b, cap=got.shape[0] + 1)
+        ok = ok and np.array_equal(got, want)
+    del host0
+    part = {"workload": f"PE 0's {nr0} reads (1/64 of config 3) partitioned into 64 owner bins, k=63",
+            "kmers": n0, "ms": part_s * 1e3, "kmers_per_s": n0 / part_s,
+            "alg_bytes": 2 * nb0 + n0 * kb, "GBps": (2 * nb0 + n0 * kb) / part_s / 1e9,
+            "bins_0_and_37_equal_oracle": bool(ok),
+            "timing": f"median of {a.steps} synchronous calls (histogram pass, host size check, sweep)"}
+    print(f"[bench] pe64: partition {n0} k-mers in {part_s * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    hit = hitter_leg(kc, d0, nb0, None, k=k, read_len=L, num_pes=P, owner_bin=-1)
+    kc.device_free(dk)
+    kc.device_free(d0)
+    # (ii) bin 0 of all reads: chunks generated on the device, bin 0 kept
+    chunk = 10_000_000
+    nchunks = (w["num_reads"] + chunk - 1) // chunk
+    cap0 = int(w["num_reads"] * (L - k + 1) / P * 1.1) + (1 << 20)
+    dbin = kc.device_alloc(cap0 * kb)
+    dch = kc.device_alloc(chunk * (L + 1))
+    n_bin, filter_s, gen_s = 0, 0.0, 0.0
+    for c in range(nchunks):
+        nr = min(chunk, w["num_reads"] - c * chunk)
+        nb = nr * (L + 1)
+        t0 = time.perf_counter()
+        kc.generate_device(_pe64_params(c * chunk, nr), dch, nb)
+        t1 = time.perf_counter()
+        m, _ = kc.partition(dev_ptr=dch, nbytes=nb, keys_out=dbin + n_bin * kb, cap=cap0 - n_bin, owner_bin=0)
+        filter_s += time.perf_counter() - t1
+        gen_s += t1 - t0
+        n_bin += m
+    kc.device_free(dch)
+    print(f"[bench] pe64: bin 0 holds {n_bin} k-mers", file=sys.stderr, flush=True)
+    kc.count_keys(dev_ptr=dbin, n=n_bin, k=k, on_device_table=True)  # warm-up
+    times = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        t = kc.count_keys(dev_ptr=dbin, n=n_bin, k=k, on_device_table=True)
+        times.append(time.perf_counter() - t0)
+    cnt_s = float(np.median(times))
+    st = kc.last_stats()
+    gpu = _table_checksum(kc, t)
+    # the oracle: the same reads regenerated on the host chunk by chunk, bin 0 kept
+    t0 = time.perf_counter()
+    parts = []
+    for c in range(nchunks):
+        nr = min(chunk, w["num_reads"] - c * chunk)
+        h = _lib.generate_reads(_pe64_params(c * chunk, nr), threads=threads)
+        cap = int(nr * (L - k + 1) / P * 1.3) + 65536  # one pass (about 1/64 of the windows)
+        try:
+            parts.append(oracle.extract_mt(h, L, k, threads, False, P, 0, cap=cap))
+        except oracle.OracleError:
+            parts.append(oracle.extract_mt(h, L, k, threads, False, P, 0))
+        del h
+    keys0 = np.concatenate(parts)
+    del parts
+    dcpu, cscpu, cnt_cpu_s = oracle.count_keys_mt(keys0, k, threads)
+    cpu_s = time.perf_counter() - t0
+    ncpu = keys0.shape[0]
+    del keys0
+    kc.device_free(dbin)
+    kc.close()
+    bin0 = {"workload": "owner bin 0 of all 1.6e8 reads (every PE's k-mers for PE 0), k=63 as 2 x u64",
+            "kmers": n_bin, "distinct": gpu[1], "count_ms": cnt_s * 1e3, "count_kmers_per_s": n_bin / cnt_s,
+            "count_alg_GBps": (4 * n_bin * kb + gpu[1] * (kb + 4)) / cnt_s / 1e9,
+            "stages_ms": {"l1": st.pass0_ms, "l2": st.part2_ms, "final": st.final_ms, "path": int(st.path)},
+            "select_ms": filter_s * 1e3, "generate_ms": gen_s * 1e3,
+            "parity": {"equal": bool(gpu == (ncpu, dcpu, cscpu)), "gpu": list(gpu), "cpu_oracle": [ncpu, dcpu, cscpu],
+                       "cpu_threads": threads, "cpu_count_s": cnt_cpu_s, "cpu_leg_s": cpu_s},
+            "timing": f"count: median of {a.steps} dakc_count_keys calls on the bin-0 keys in HBM; select: "
+                      "the bin-0 filter over 16 chunks of reads (the k-mers other PEs would send)"}
+    out = {"metric": "k-mers counted/s (one PE's share of a 64-PE run)", "value": n_bin / cnt_s,
+           "unit": "k-mers/s", "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": cnt_s * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64x2",
+           "data": "synthetic (seeded, generated on the device, regenerated on the host for the oracle)",
+           "config": {"workload": "config 3: 2 Gbp genome, 250 bp at 20x, 1% subs, 0.1% N runs, k=63, P=64",
+                      "parallelism": "single GPU (one PE's share)"},
+           "partition": part, "bin0": bin0, "hitter": hit}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="genome", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="genome", choices=sorted(WORKLOADS) + ["pe64"],
+                    help="pe64: config 3, one PE's share of a 64-PE run (its own line)")
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"],
                     help="plain: one job per count (the headline); batch: many read sets per "
                          "captured launch vs one launch per set; serve: Poisson-arrival requests, "
@@ -526,6 +679,7 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the whole-workload check of the GPU table against the oracle's threaded count")
     ap.add_argument("--no-batched", action="store_true", help="skip the bounded batched sub-object")
+    ap.add_argument("--no-hitter", action="store_true", help="skip the HITTER=1 sub-object")
     ap.add_argument("--no-served", action="store_true", help="skip the bounded served sub-object")
     ap.add_argument("--line-batch-reads", type=float, default=4e6,
                     help="plain mode's batched sub-object: read sets taken up to this many reads")
@@ -543,6 +697,8 @@ def main() -> int:
 
     from dakc_amd import KmerCounter, _lib
 
+    if a.workload == "pe64":
+        return bench_pe64(a)
     if a.mode == "batch":
         return bench_batch(a)
     if a.mode == "serve":
@@ -617,6 +773,13 @@ def main() -> int:
     if not a.no_parity:
         print("[bench] parity: whole table against the oracle's threaded count", file=sys.stderr, flush=True)
         parity = full_parity(kc, t, w, dreads, nbytes, threads)
+    hit = None
+    if not a.no_hitter:
+        print("[bench] hitter: the same reads with per-bucket pre-aggregation", file=sys.stderr, flush=True)
+        csum = (parity["gpu"]["n"], parity["gpu"]["d"], parity["gpu"]["checksum"]) if parity else None
+        hit = hitter_leg(kc, dreads, nbytes, csum, k=w["k"], read_len=w["read_len"],
+                         canonical=w.get("canonical", False))
+        t = kc.count_device(dreads, nbytes)  # the HITTER count used the workspace: count again
     post = post_count(kc, t, kb) if not a.no_post else None  # changes the table: after the parity check
     pre = pre_count(kc, dreads, w) if not a.no_post else None
     stages = {
@@ -651,6 +814,7 @@ def main() -> int:
                      "avg_ms": dom_ms, "alg_bytes_per_launch": dom_bytes},
         "stages": stages,
         "parity": parity,
+        "hitter": hit,
         "pre_count": pre,
         "post_count": post,
     }

@@ -799,6 +799,7 @@ int dko_count_hitter(const char *reads, size_t nbytes, int read_len, int k, int 
     memset(st, 0, sizeof(*st));
     if (k < 1 || k > 64 || num_pes < 1 || read_len < 1) return DKO_E_INVAL;
     if (num_pes > 1 && owner_bin >= (int32_t)num_pes) return DKO_E_INVAL;
+    if (bucket_size == 0) bucket_size = 10000; /* KCOUNT_BUCKET_SIZE, dakc/Makefile:11 */
     /* the while condition of read_till_buf_max compares with bucket_size -
      * READLEN: below 0 it would never stop filling the buffer */
     if (bucket_size <= (uint32_t)read_len) return DKO_E_INVAL;

@@ -112,7 +112,7 @@ void dko_free(void *p);
  * HITTER=0 table.  With num_pes > 1 and owner_bin == -1 both are the
  * concatenation of the P owners' tables (offsets: num_pes + 1 entries each,
  * may be NULL); with owner_bin >= 0 only that owner's.  Tables are malloc'd
- * (dko_free).  bucket_size must exceed read_len. */
+ * (dko_free).  bucket_size (0: the reference's 10000) must exceed read_len. */
 typedef struct {
     uint64_t buckets;      /* buckets flushed (with at least one k-mer) */
     uint64_t normal_keys;  /* plain k-mers sent */

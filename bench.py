@@ -573,8 +573,11 @@ def bench_pe64(a) -> int:
     ok = same_in and n0b == n0
     for b in (0, 37):
         got = kc.d2h(dk + int(offs[b]) * kb, int(offs[b + 1] - offs[b]) * kb, np.uint64).reshape(-1, 2)
-        want = oracle.extract_mt(host0, L, k, threads, False, P, b, cap=got.shape[0] + 1)
-        ok = ok and np.array_equal(got, want)
+        try:
+            want = oracle.extract_mt(host0, L, k, threads, False, P, b, cap=got.shape[0] + 1)
+        except oracle.OracleError:  # the oracle's bin is larger than the GPU's
+            want = None
+        ok = ok and want is not None and np.array_equal(got, want)
     del host0
     part = {"workload": f"PE 0's {nr0} reads (1/64 of config 3) partitioned into 64 owner bins, k=63",
             "kmers": n0, "ms": part_s * 1e3, "kmers_per_s": n0 / part_s,

@@ -390,6 +390,13 @@ int plan_choose_path(Plan *pl, int algo) {
   while ((owners << (b1max + 1)) <= 1024 && b1max < 10) ++b1max;
   uint32_t b1 = std::min(std::min(b1max, (b12 + 1) / 2), bits);
   if (b12 > b1 + 10) b1 = std::min(std::min(b1max, b12 - 10), bits);
+  // the tuned first-level width: one bit more makes the final items gather
+  // from half as many second-level groups (longer segments), one bit less
+  // the reverse; the second level keeps the rest
+  if (p.tune_b1_shift) {
+    const int64_t t = (int64_t)b1 + p.tune_b1_shift;
+    b1 = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(t, std::min(b1max, std::min(bits, b12))));
+  }
   const uint32_t b2 = std::min(std::min<uint32_t>(10, b12 > b1 ? b12 - b1 : 0), bits - b1);
   if (b12 > b1 + b2) return 0;  // sub-buckets would be too large on average
   // L1 tile geometry: a record's slots take ceil(S / items) threads (a key

@@ -34,6 +34,7 @@ void apply_tuning(const dakc_ctx *c, Plan *pl, int algo) {
   }
   pl->tune_fin_nt = e.fin_nt;
   pl->tune_fill_pct = e.fill_pct;
+  pl->tune_b1_shift = e.b1_shift;
   pl->num_cus = (uint32_t)std::max(c->num_cus, 1);
   plan_choose_path(pl, algo == DAKC_ALGO_AUTO && e.algo == 1 ? DAKC_ALGO_LSD : algo);
 }
@@ -358,11 +359,13 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
   if (s) return s;
   struct Cand {
     uint32_t fin, fill, algo;
+    int32_t b1_shift;
   };
   // the default geometry first: another candidate replaces the best so far
   // only when it is more than 5% faster (the median of `trials` timed runs)
-  const Cand cands[] = {{1024, 90, 0}, {1024, 75, 0}, {1024, 60, 0}, {512, 90, 0},
-                        {512, 75, 0},  {512, 60, 0},  {256, 90, 0},  {1024, 90, 1}};
+  const Cand cands[] = {{1024, 90, 0, 0}, {1024, 75, 0, 0}, {1024, 60, 0, 0}, {1024, 90, 0, 1},
+                        {1024, 75, 0, 1}, {512, 90, 0, 0},  {512, 75, 0, 0},  {512, 60, 0, 0},
+                        {256, 90, 0, 0},  {1024, 90, 1, 0}};
   TuneEntry best;
   best.ms = 1e300;
   dakc_table t;
@@ -383,6 +386,7 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
       e.fin_nt = cd.fin;
       e.fill_pct = cd.fill;
       e.algo = cd.algo;
+      e.b1_shift = cd.b1_shift;
       ctx->tuning[{base.kw, size_class(base.nmax)}] = e;
     }
     std::vector<double> ms;
@@ -407,6 +411,7 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
       best.fin_nt = cd.fin;
       best.fill_pct = cd.fill;
       best.algo = cd.algo;
+      best.b1_shift = cd.b1_shift;
       best.ms = med;
     }
   }
@@ -430,9 +435,10 @@ extern "C" int dakc_tuning_save(dakc_ctx *ctx, const char *path) {
   size_t i = 0;
   for (auto &kv : ctx->tuning) {
     std::fprintf(f, "  {\"key_words\": %d, \"size_class\": %d, \"fin_nt\": %u, \"fill_pct\": %u, "
-                 "\"algo\": \"%s\", \"ms\": %.4f}%s\n",
+                 "\"algo\": \"%s\", \"ms\": %.4f, \"b1_shift\": %d}%s\n",
                  kv.first.first, kv.first.second, kv.second.fin_nt, kv.second.fill_pct,
-                 kv.second.algo ? "lsd" : "msd", kv.second.ms, ++i < ctx->tuning.size() ? "," : "");
+                 kv.second.algo ? "lsd" : "msd", kv.second.ms, kv.second.b1_shift,
+                 ++i < ctx->tuning.size() ? "," : "");
   }
   std::fprintf(f, " ]\n}\n");
   std::fclose(f);
@@ -450,15 +456,18 @@ extern "C" int dakc_tuning_load(dakc_ctx *ctx, const char *path) {
     unsigned fin, fill;
     char algo[8] = {0};
     double ms;
-    if (std::sscanf(line, " {\"key_words\": %d, \"size_class\": %d, \"fin_nt\": %u, \"fill_pct\": %u, "
-                          "\"algo\": \"%3[a-z]\", \"ms\": %lf}",
-                    &kw, &cls, &fin, &fill, algo, &ms) == 6) {
+    int shift = 0;
+    const int got = std::sscanf(line, " {\"key_words\": %d, \"size_class\": %d, \"fin_nt\": %u, \"fill_pct\": %u, "
+                                      "\"algo\": \"%3[a-z]\", \"ms\": %lf, \"b1_shift\": %d}",
+                                &kw, &cls, &fin, &fill, algo, &ms, &shift);
+    if (got >= 6) {  // files without b1_shift (6 fields) keep the planner's width
       TuneEntry e;
       e.fin_nt = fin == 256 ? 256 : fin == 1024 ? 1024 : 512;
       // above 100 the items outgrow LDS and are counted digit by digit
       e.fill_pct = std::min(400u, std::max(30u, fill));
       e.algo = std::strcmp(algo, "lsd") == 0 ? 1 : 0;
       e.ms = ms;
+      e.b1_shift = got == 7 ? std::max(-2, std::min(2, shift)) : 0;
       ctx->tuning[{kw, cls}] = e;
     }
   }

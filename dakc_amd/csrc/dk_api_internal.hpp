@@ -74,6 +74,7 @@ struct Plan {
   // tunables set before plan_choose_path (autotuner; defaults otherwise),
   // and the CUs of the device the job runs on
   uint32_t tune_fin_nt = 1024, tune_fill_pct = 90, num_cus = 256;
+  int32_t tune_b1_shift = 0;  // first-level key bits over (or under) the planner's choice
 };
 
 // path choice (dakc_opts.algo)
@@ -188,6 +189,7 @@ namespace dk {
 // autotuned MSD parameters for one (key width, job-size class)
 struct TuneEntry {
   uint32_t fin_nt = 1024, fill_pct = 90, algo = 0;  // algo: 0 MSD, 1 LSD
+  int32_t b1_shift = 0;                             // first-level digit width - the planner's
   double ms = 0;
 };
 int size_class(uint64_t nmax);

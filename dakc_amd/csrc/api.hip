@@ -433,9 +433,9 @@ int plan_choose_path(Plan *pl, int algo) {
     p.ipbc[c] = (nd2 + pk - 1) / pk;
     packmax = std::max(packmax, pk);
   }
-  const uint32_t nb3 = fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup
+  const uint32_t nb3 = fin_nt >= 1024 ? 2048 : fin_nt >= 512 ? 1024 : 512;  // LDS bins of a final workgroup
   uint32_t b3 = 0;
-  while (b3 < 10 && b3 < bits - b1 - b2f && (packmax << (b3 + 1)) <= nb3) ++b3;
+  while (b3 < 11 && b3 < bits - b1 - b2f && (packmax << (b3 + 1)) <= nb3) ++b3;
   p.b1 = b1;
   p.b2 = b2f;
   p.b3 = b3;

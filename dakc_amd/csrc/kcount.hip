@@ -1459,14 +1459,14 @@ template <typename KeyT, int NT, int ITEMS, int NB3, int HT, bool ONE>
 __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   constexpr int CAP = NT * ITEMS;
   constexpr int TPT = FIN_MAX_GROUPS / NT;
-  __shared__ KeyT B[CAP];
+  __shared__ KeyT B[CAP + 8];  // the rank reads up to 7 keys past a bin's end
   __shared__ union {
     GatherSmem<FIN_MAX_GROUPS, CAP> seg;
     uint16_t hp[CAP];
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
   __shared__ uint32_t s_mixed[NB3 / 32];
-  __shared__ uint32_t s_wave[NT / 64], s_total, s_sb, s_job, s_fail, s_next;
+  __shared__ uint32_t s_wave[NT / 64], s_wmin[NT / 64], s_total, s_sb, s_job, s_fail, s_next;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x;
 
@@ -1555,11 +1555,9 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
     __syncthreads();
 
-    // Order each bin by counting: a key's place is the number of keys of its
-    // bin that are smaller, plus the equal ones stored before it; reads are
-    // independent and copies of one k-mer keep their order.  Bins whose keys
-    // are all equal are already in order; a mixed bin longer than RANK_MAX flags
-    // the job, which is then recounted on the LSD path.
+    // Bins whose keys are not all equal (mixed) are ranked below; a mixed bin
+    // longer than RANK_MAX flags the job, which is then recounted on the LSD
+    // path.  Keys are reloaded in B's order (position tid + q * NT).
     constexpr uint32_t RANK_MAX = 128;
   #pragma unroll
     for (int q = 0; q < ITEMS; ++q) {
@@ -1572,60 +1570,113 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       }
     }
     __syncthreads();
-    uint32_t newpos[ITEMS];
+    // The place of a key in sorted order is its bin start + the keys of its
+    // bin that precede it (smaller, or equal and stored before it); a bin of
+    // copies of one k-mer keeps its order.  s_map[place] = position in B.
+    uint16_t *s_map = s_u.hp;
     bool bad = false;
   #pragma unroll
     for (int q = 0; q < ITEMS; ++q) {
       const uint32_t i = tid + q * NT;
-      newpos[q] = i;
-      if (i < mcnt && ((s_mixed[dig[q] >> 5] >> (dig[q] & 31)) & 1)) {
+      if (i < mcnt) {
         const KeyT x = keys[q];
         const uint32_t lo = s_h[dig[q]], hi = s_h[dig[q] + 1];
-        if (hi - lo <= RANK_MAX) {
-          // eight independent LDS reads per step; past the bin end y = x, which
-          // counts neither as smaller nor as an earlier copy (t > i there)
-          uint32_t r = 0;
-          for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
-            KeyT y[8];
+        uint32_t place = i;
+        if ((s_mixed[dig[q] >> 5] >> (dig[q] & 31)) & 1) {
+          if (hi - lo <= RANK_MAX) {
+            // four independent LDS reads per step; the keys read past the
+            // bin's end (inside B's padding at most) are masked out
+            uint32_t r = 0;
+            for (uint32_t t0 = lo; t0 < hi; t0 += 4) {
+              KeyT y[4];
   #pragma unroll
-            for (int j = 0; j < 8; ++j) y[j] = t0 + j < hi ? B[t0 + j] : x;
+              for (int j = 0; j < 4; ++j) y[j] = B[t0 + j];
   #pragma unroll
-            for (int j = 0; j < 8; ++j) r += key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x));
+              for (int j = 0; j < 4; ++j)
+                r += (t0 + j < hi && (key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x)))) ? 1u : 0u;
+            }
+            place = lo + r;
+          } else {
+            bad = true;
           }
-          newpos[q] = lo + r;
-        } else {
-          bad = true;
+        }
+        s_map[place] = (uint16_t)i;
+      }
+    }
+    if (bad) atomicOr(err, ERR_OVERFLOW);  // recounted on the LSD path
+    __syncthreads();
+
+    // Runs in place order, places tid * ITEMS .. + ITEMS - 1 per thread: a
+    // place starts a run (a head) when its key differs from the previous
+    // place's (keys of different bins differ).  A run ends at the next head:
+    // in this thread's places, else at the first head of a later thread
+    // (a suffix minimum over the threads), else at mcnt.
+    uint32_t heads = 0, hmask = 0, first = 0xFFFFFFFFu;
+    {
+      const uint32_t p0 = tid * ITEMS;
+      KeyT prev{};
+      // a place's word is < mcnt unless a bin went unranked (then the job
+      // is recounted); the clamp keeps such reads inside B
+      auto at = [&](uint32_t pl) -> KeyT {
+        const uint32_t sp = s_map[pl];
+        return B[sp < mcnt ? sp : 0];
+      };
+      if (p0 > 0 && p0 < mcnt) prev = at(p0 - 1);
+  #pragma unroll
+      for (int q = 0; q < ITEMS; ++q) {
+        const uint32_t pl = p0 + q;
+        if (pl < mcnt) {
+          const KeyT x = at(pl);
+          const bool h = pl == 0 || !key_eq(x, prev);
+          hmask |= (h ? 1u : 0u) << q;
+          heads += h;
+          if (h && first == 0xFFFFFFFFu) first = pl;
+          prev = x;
         }
       }
     }
-    if (bad) atomicOr(err, ERR_OVERFLOW);
-    __syncthreads();
+    // the prefix of the head counts and the suffix minimum of the first heads
+    uint32_t o = 0, after = 0;
+    {
+      const uint32_t lane = tid & 63, wave = tid >> 6;
+      uint32_t x = heads, mn = first;
   #pragma unroll
-    for (int q = 0; q < ITEMS; ++q)
-      if (tid + q * NT < mcnt) B[newpos[q]] = keys[q];
-    __syncthreads();
-
-    // run-length reduce inside LDS
-    uint32_t heads = 0, hmask = 0;
-  #pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-      const uint32_t i = tid * ITEMS + q;
-      const bool h = i < mcnt && (i == 0 || !key_eq(B[i], B[i - 1]));
-      hmask |= (h ? 1u : 0u) << q;
-      heads += h;
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64), z = __shfl_down(mn, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+        if (lane + d < 64) mn = min(mn, z);
+      }
+      if (lane == 63) s_wave[wave] = x;
+      if (lane == 0) s_wmin[wave] = mn;
+      const uint32_t nx = __shfl_down(mn, 1, 64);
+      __syncthreads();
+      uint32_t pre = 0, later = lane == 63 ? 0xFFFFFFFFu : nx;
+      for (uint32_t w = 0; w < NT / 64; ++w) {
+        const uint32_t sw = s_wave[w], mw = s_wmin[w];
+        if (w < wave) pre += sw;
+        if (w > wave) later = min(later, mw);
+      }
+      if (tid == NT - 1) s_total = pre + x;
+      o = pre + x - heads;
+      after = later == 0xFFFFFFFFu ? mcnt : later;
+      __syncthreads();
     }
-    uint32_t o = scan_block<NT, false>(heads, s_wave, &s_total);
     const uint32_t H = s_total;
-    uint16_t *s_hp = s_u.hp;
-  #pragma unroll
-    for (int q = 0; q < ITEMS; ++q)
-      if ((hmask >> q) & 1) s_hp[o++] = (uint16_t)(tid * ITEMS + q);
     const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, n_dev, err, &s_excl);
-    for (uint32_t h = tid; h < H; h += NT) {
-      const uint32_t i = s_hp[h];
-      const uint32_t end = h + 1 < H ? s_hp[h + 1] : mcnt;
-      out_keys[excl + h] = B[i];
-      out_counts[excl + h] = end - i;
+    {
+      uint32_t end = after;  // the next head after the place at hand
+      o += heads;
+  #pragma unroll
+      for (int q = ITEMS - 1; q >= 0; --q) {
+        if ((hmask >> q) & 1) {
+          const uint32_t pl = tid * ITEMS + q;
+          --o;
+          const uint32_t sp = s_map[pl];
+          out_keys[excl + o] = B[sp < mcnt ? sp : 0];
+          out_counts[excl + o] = end - pl;
+          end = pl;
+        }
+      }
     }
     return false;
   };
@@ -1840,8 +1891,9 @@ template <typename KeyT, int ITEMS, bool ONE>
 static void msd_levels_nt(uint32_t fin_nt, const BatchArgs &a, const JobDesc &one, uint32_t n1,
                           uint32_t n2, uint32_t n3, hipStream_t st) {
   constexpr bool W1 = sizeof(KeyT) == 8;
-  // hash slots (12 bytes each) filling the final level's key buffer
-  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 1024, W1 ? 10240 : 0, ONE>(a, one, n1, n2, n3, st);
+  // hash slots (12 bytes each) filling the final level's key buffer; LDS bins
+  // of a final workgroup as plan_choose_path assumes (fin_bins)
+  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 2048, W1 ? 10240 : 0, ONE>(a, one, n1, n2, n3, st);
   else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 5120 : 0, ONE>(a, one, n1, n2, n3, st);
   else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2560 : 0, ONE>(a, one, n1, n2, n3, st);
 }

@@ -844,7 +844,7 @@ int dko_count_hitter(const char *reads, size_t nbytes, int read_len, int k, int 
      * every other run is a light entry */
     const int all = num_pes > 1 && owner_bin < 0;
     const uint32_t P = all ? num_pes : 1;
-    qsort(s.pairs, (size_t)s.np, sizeof(hit_pair), hit_pair_cmp);
+    if (s.np) qsort(s.pairs, (size_t)s.np, sizeof(hit_pair), hit_pair_cmp);  /* no pairs: s.pairs may be NULL */
     uint64_t hd = 0;  /* merged heavy pairs, in place */
     for (uint64_t i = 0; i < s.np; i++) {
         if (hd && s.pairs[hd - 1].key == s.pairs[i].key && s.pairs[hd - 1].owner == s.pairs[i].owner)

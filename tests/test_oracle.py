@@ -238,3 +238,27 @@ def test_owner_seed_matches_bruteforce():
         o = oracle.count(data, 60, k, num_pes=5, owner_bin=-1, seed=0x5EED)
         assert np.array_equal(a[0], z[0]) and np.array_equal(a[3], z[3])
         assert a[3][-1] > 50 and not np.array_equal(a[3], o[3])
+
+
+@pytest.mark.parametrize("target", ["oracle_asan", "oracle_tsan"])
+def test_sanitized(target, tmp_path):
+    """Every oracle entry point over messy records (tests/oracle_sanitize.c),
+    built with AddressSanitizer + UndefinedBehaviorSanitizer and, for the
+    threaded count and extraction, ThreadSanitizer: no report, and the
+    cross-checks between the entry points hold."""
+    import os
+    import subprocess
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    out = tmp_path / "_build"
+    r = subprocess.run(["make", "-s", "-f", str(root / "oracle" / "Makefile"), f"OUT={out}", str(out / target)],
+                       capture_output=True, text=True)
+    if r.returncode != 0 and "sanitize" in r.stderr and "cannot find" in r.stderr:
+        pytest.skip(f"{target}: sanitizer runtime not installed")
+    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1", UBSAN_OPTIONS="halt_on_error=1",
+               TSAN_OPTIONS="halt_on_error=1")
+    run = subprocess.run([str(out / target)], capture_output=True, text=True, env=env, timeout=600)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "all checks passed" in run.stdout

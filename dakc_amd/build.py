@@ -5,6 +5,7 @@
 Outputs:
     dakc_amd/libdakc_amd.so        the product: C ABI of include/dakc.h
     dakc_amd/dakc                  CLI driver (file in, k, table out)
+    dakc_amd/hpp_example           smallest driver of include/dakc.hpp
     oracle/_build/libdakc_oracle.so  CPU oracle (test infrastructure)
 """
 from __future__ import annotations
@@ -21,6 +22,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libdakc_amd.so"
 CLI = PKG / "dakc"
+HPP_EXAMPLE = PKG / "hpp_example"
 ORACLE_LIB = ROOT / "oracle" / "_build" / "libdakc_oracle.so"
 
 HIP_SOURCES = ["kcount.hip", "api.hip", "serve.hip", "batch.hip", "post.hip", "input.hip", "noalloc.hip", "phases.hip", "hitter.hip"]
@@ -66,10 +68,11 @@ def build_lib(verbose: bool = False) -> Path:
 
 
 def build_cli(verbose: bool = False) -> Path:
-    tmp = CLI.with_name("dakc.tmp")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(PKG / "cli" / "dakc_main.cpp"),
-          "-o", str(tmp), f"-L{PKG}", "-ldakc_amd", "-Wl,-rpath,$ORIGIN"], verbose)
-    os.replace(tmp, CLI)
+    for src, dst in (("dakc_main.cpp", CLI), ("hpp_example.cpp", HPP_EXAMPLE)):
+        tmp = dst.with_name(dst.name + ".tmp")
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(PKG / "cli" / src),
+              "-o", str(tmp), f"-L{PKG}", "-ldakc_amd", "-Wl,-rpath,$ORIGIN"], verbose)
+        os.replace(tmp, dst)
     return CLI
 
 

@@ -350,8 +350,12 @@ int make_plan_keys(uint64_t n, int k, uint32_t num_pes, uint64_t seed, Plan *pl)
   return DAKC_OK;
 }
 
-// MSD geometry.  A final item should hold at most tune_fill_pct (90 by
-// default) percent of what one final workgroup sorts in LDS on average.  The
+// MSD geometry.  A final item should hold at most tune_fill_pct percent of
+// what one final workgroup sorts in LDS on average (by default 90 for
+// one-word keys and 75 for two-word keys: the k = 55 genome count's final
+// level took 40.4 ms at 90, where sub-buckets over LDS capacity are redone
+// digit by digit and hold up the look-back of every later item, and 27.2 ms
+// at 75; the k = 31 count's took 17.3 ms at 90 and 20.2 ms at 75).  The
 // coarsest split that meets it fixes b1 (<= 10: 1024 first-level buckets,
 // fewer key bits when owner bins share them); the second level then splits up
 // to two bits finer (b2 <= 10) and a final item takes `pack` consecutive
@@ -375,7 +379,7 @@ int plan_choose_path(Plan *pl, int algo) {
   // owner filter keeps about 1/P of the slots (MurmurHash64A spreads them)
   const uint64_t nexp = p.owner_bin >= 0 && p.num_pes > 1 ? (p.nmax + p.num_pes - 1) / p.num_pes : p.nmax;
   const uint64_t target = std::min<uint64_t>(
-      cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : 90) / 100,
+      cap3 * (uint64_t)(p.tune_fill_pct ? p.tune_fill_pct : p.kw == 2 ? 75 : 90) / 100,
       std::max<uint64_t>(cap3 / 8, nexp / std::max(p.num_cus, 1u)));
   // With all P owner bins counted, the owner is the top of the first-level
   // digit: MurmurHash64A spreads the k-mers evenly over the owners, so each

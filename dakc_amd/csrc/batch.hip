@@ -363,9 +363,10 @@ extern "C" int dakc_autotune(dakc_ctx *ctx, const dakc_job *job, int trials, dou
   };
   // the default geometry first: another candidate replaces the best so far
   // only when it is more than 5% faster (the median of `trials` timed runs)
-  const Cand cands[] = {{1024, 90, 0, 0}, {1024, 75, 0, 0}, {1024, 60, 0, 0}, {1024, 90, 0, 1},
-                        {1024, 75, 0, 1}, {512, 90, 0, 0},  {512, 75, 0, 0},  {512, 60, 0, 0},
-                        {256, 90, 0, 0},  {1024, 90, 1, 0}};
+  // (fill 0: the key width's default, 90 or 75)
+  const Cand cands[] = {{1024, 0, 0, 0},  {1024, 90, 0, 0}, {1024, 75, 0, 0}, {1024, 60, 0, 0},
+                        {1024, 0, 0, 1},  {512, 90, 0, 0},  {512, 75, 0, 0},  {512, 60, 0, 0},
+                        {256, 90, 0, 0},  {1024, 0, 1, 0}};
   TuneEntry best;
   best.ms = 1e300;
   dakc_table t;
@@ -463,8 +464,9 @@ extern "C" int dakc_tuning_load(dakc_ctx *ctx, const char *path) {
     if (got >= 6) {  // files without b1_shift (6 fields) keep the planner's width
       TuneEntry e;
       e.fin_nt = fin == 256 ? 256 : fin == 1024 ? 1024 : 512;
-      // above 100 the items outgrow LDS and are counted digit by digit
-      e.fill_pct = std::min(400u, std::max(30u, fill));
+      // above 100 the items outgrow LDS and are counted digit by digit; 0 is
+      // the key width's default
+      e.fill_pct = fill == 0 ? 0u : std::min(400u, std::max(30u, fill));
       e.algo = std::strcmp(algo, "lsd") == 0 ? 1 : 0;
       e.ms = ms;
       e.b1_shift = got == 7 ? std::max(-2, std::min(2, shift)) : 0;

@@ -73,7 +73,7 @@ struct Plan {
   uint32_t RT1 = 1, wpr1 = 0, ntiles1 = 0, tcap1 = 0, G = 1, ngroups = 0, cap2 = 0, fin_nt = 256;
   // tunables set before plan_choose_path (autotuner; defaults otherwise),
   // and the CUs of the device the job runs on
-  uint32_t tune_fin_nt = 1024, tune_fill_pct = 90, num_cus = 256;
+  uint32_t tune_fin_nt = 1024, tune_fill_pct = 0, num_cus = 256;  // fill 0: the key width's default
   int32_t tune_b1_shift = 0;  // first-level key bits over (or under) the planner's choice
 };
 
@@ -188,7 +188,7 @@ struct ServeState;  // served mode (serve.hip)
 namespace dk {
 // autotuned MSD parameters for one (key width, job-size class)
 struct TuneEntry {
-  uint32_t fin_nt = 1024, fill_pct = 90, algo = 0;  // algo: 0 MSD, 1 LSD
+  uint32_t fin_nt = 1024, fill_pct = 0, algo = 0;  // fill 0: the key width's default; algo: 0 MSD, 1 LSD
   int32_t b1_shift = 0;                             // first-level digit width - the planner's
   double ms = 0;
 };

@@ -1450,6 +1450,161 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   return true;
 }
 
+// Distinct-key path of the final level for two-word keys (k = 33..64): the
+// gathered keys go into an LDS table of HT slots of (claim word, hi, count).
+// A key's claim word is lo ^ hi * C (C odd, so lo = word ^ hi * C): it mixes
+// both words, so keys that share a low word (k-mers that differ only in their
+// first 32 - k bases, as substitution errors often make) still differ in it.
+// First every key walks to the slot holding its claim word or claims an
+// empty one by a CAS, and a claimer writes its high word; after a barrier
+// every key compares its slot's high word with its own and adds 1 to the
+// count (a mismatch, two keys with one claim word, sends the item to the
+// other path).  As in fin_distinct the first probes of all keys go out back
+// to back; the occupied slots are split by the next b3 bits through a 16-bit
+// slot index per key in bin order, the distinct keys of each bin are ranked
+// by counting, and each pair is written to its place.
+// Returns false, having changed nothing the other path needs, when a key
+// finds no slot within 32 probes or its claim word is the empty word.
+constexpr uint64_t K2_MIX = 0x9E3779B97F4A7C15ull;
+
+DK_DEV uint64_t k2_claim(const K2 &x) { return x.lo ^ (x.hi * K2_MIX); }
+
+template <int NT, int ITEMS, int HT>
+DK_DEV bool fin_distinct2(const JobDesc &J, const MsdArgs &m, const K2 (&keys)[ITEMS], uint32_t mine,
+                          uint8_t *lds, uint32_t *s_h, uint32_t *s_wave, uint32_t *s_total, uint32_t *s_fail,
+                          uint64_t *s_excl, uint32_t sb, uint32_t nsb, const FinItem &fi) {
+  constexpr int SPT = HT / NT;  // table slots per thread
+  const uint32_t tid = opaque_tid();
+  uint64_t *kc = reinterpret_cast<uint64_t *>(lds), *khi = kc + HT;
+  uint32_t *tc = reinterpret_cast<uint32_t *>(khi + HT);
+  uint16_t *ix = reinterpret_cast<uint16_t *>(tc + HT);
+  uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    kc[tid + j * NT] = HASH_EMPTY;
+    tc[tid + j * NT] = 0;
+  }
+  __syncthreads();
+  if (*s_fail) return false;
+  bool lost = false;
+  uint32_t hh[ITEMS];
+  uint64_t old[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      const uint64_t c = k2_claim(keys[u]);
+      lost |= c == HASH_EMPTY;
+      hh[u] = (uint32_t)((((c * 0xC2B2AE3D27D4EB4Full) >> 32) * HT) >> 32);  // [0, HT)
+      old[u] = atomicCAS((unsigned long long *)&kc[hh[u]], HASH_EMPTY, c);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      const uint64_t c = k2_claim(keys[u]);
+      uint32_t h = hh[u];
+      uint64_t o = old[u];
+      for (int pr = 0; pr < 32 && o != HASH_EMPTY && o != c; ++pr) {
+        h = h + 1 == (uint32_t)HT ? 0u : h + 1;
+        o = atomicCAS((unsigned long long *)&kc[h], HASH_EMPTY, c);
+      }
+      if (o == HASH_EMPTY) khi[h] = keys[u].hi;  // claimed here
+      else lost |= o != c;
+      hh[u] = h;
+    }
+  }
+  __syncthreads();  // every claimer's high word is written
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if ((uint32_t)u < mine) {
+      if (khi[hh[u]] == keys[u].hi) atomicAdd(&tc[hh[u]], 1u);
+      else lost = true;
+    }
+  }
+  if (lost) *s_fail = 1;
+  __syncthreads();
+  const bool fits = !*s_fail;
+  if (tid == 0) atomicAdd(hc + (fits ? 1 : 0), 1u);
+  if (!fits) {
+    __syncthreads();  // everyone has read s_fail before the other path reuses LDS
+    return false;
+  }
+  // the keys of the occupied slots
+  auto key_at = [&](uint32_t sl) {
+    const uint64_t hi = khi[sl];
+    return K2{kc[sl] ^ (hi * K2_MIX), hi};
+  };
+  // split the occupied slots by the next b3 bits: ix[p] is the slot of the
+  // p-th key in bin order, and the keys stay where they are
+  const uint32_t nd3 = fi.nbins;
+  {
+    uint32_t ed[SPT], er[SPT];
+    uint32_t occ = 0;
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (kc[i] != HASH_EMPTY) {
+        occ |= 1u << j;
+        ed[j] = fin_bin(m, fi, key_at(i));
+        er[j] = atomicAdd(&s_h[ed[j]], 1u);
+      }
+    }
+    __syncthreads();
+    scan_lds<NT, false>(s_h, nd3, s_wave, s_total);
+#pragma unroll
+    for (int j = 0; j < SPT; ++j)
+      if ((occ >> j) & 1) ix[s_h[ed[j]] + er[j]] = (uint16_t)(tid + j * NT);
+  }
+  const uint32_t H = s_h[nd3];
+  constexpr uint32_t epoch = 1;
+  if (tid == 0) lb_store(J.flags + sb, lb_pack(epoch, sb == 0 ? LB_INC : LB_AGG, H));  // early
+  __syncthreads();
+  // place = bin start + number of smaller keys in the bin
+  uint32_t place[SPT];
+  bool big = false;
+#pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    const uint32_t p = tid + i * NT;
+    place[i] = 0;
+    if (p < H) {
+      const uint32_t sl = ix[p];
+      const K2 x = key_at(sl);
+      const uint32_t d = fin_bin(m, fi, x);
+      const uint32_t lo = s_h[d], hi = s_h[d + 1];
+      uint32_t r = 0;
+      if (hi - lo > 1) {
+        if (hi - lo > 256) {
+          big = true;
+        } else {
+          for (uint32_t t0 = lo; t0 < hi; t0 += 4) {
+            K2 y[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              y[q] = key_at(t0 + q < hi ? ix[t0 + q] : sl);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r += key_lt(y[q], x) ? 1u : 0u;
+          }
+        }
+      }
+      place[i] = lo + r;
+    }
+  }
+  if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
+  const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, J.err, s_excl);
+  K2 *out_keys = reinterpret_cast<K2 *>(J.out1);
+#pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    const uint32_t p = tid + i * NT;
+    if (p < H) {
+      const uint32_t sl = ix[p];
+      out_keys[excl + place[i]] = key_at(sl);
+      J.tcounts[excl + place[i]] = tc[sl];
+    }
+  }
+  return true;
+}
+
 // Final: one sub-bucket (bucket b, digit s) gathered from the L2 groups of b
 // into LDS, split by the next b3 bits, each bin ordered by counting, run-length
 // reduced, and written at the place the decoupled look-back gives.
@@ -1536,6 +1691,12 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
                                                          s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
                                                          nsb, fi))
+        return false;
+    } else {
+      static_assert((size_t)HT * 22 <= sizeof(B) && HT % NT == 0 && HT <= 65536,
+                    "hash table does not fit the key buffer");
+      if (fin_distinct2<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint8_t *>(B), s_h, s_wave, &s_total,
+                                       &s_fail, &s_excl, sb, nsb, fi))
         return false;
     }
 
@@ -1891,11 +2052,12 @@ template <typename KeyT, int ITEMS, bool ONE>
 static void msd_levels_nt(uint32_t fin_nt, const BatchArgs &a, const JobDesc &one, uint32_t n1,
                           uint32_t n2, uint32_t n3, hipStream_t st) {
   constexpr bool W1 = sizeof(KeyT) == 8;
-  // hash slots (12 bytes each) filling the final level's key buffer; LDS bins
+  // hash slots (12 bytes each for one-word keys, 22 for two-word keys) in the
+  // final level's key buffer; LDS bins
   // of a final workgroup as plan_choose_path assumes (fin_bins)
-  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 2048, W1 ? 10240 : 0, ONE>(a, one, n1, n2, n3, st);
-  else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 5120 : 0, ONE>(a, one, n1, n2, n3, st);
-  else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2560 : 0, ONE>(a, one, n1, n2, n3, st);
+  if (fin_nt == 1024) msd_levels<KeyT, ITEMS, 1024, 2048, W1 ? 10240 : 5120, ONE>(a, one, n1, n2, n3, st);
+  else if (fin_nt == 512) msd_levels<KeyT, ITEMS, 512, 1024, W1 ? 5120 : 2560, ONE>(a, one, n1, n2, n3, st);
+  else msd_levels<KeyT, ITEMS, 256, 512, W1 ? 2560 : 1280, ONE>(a, one, n1, n2, n3, st);
 }
 
 void launch_msd_batch(int kw, uint32_t fin_nt, const BatchArgs &a, const JobDesc *one, uint32_t n1,

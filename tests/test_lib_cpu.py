@@ -133,3 +133,18 @@ def test_error_bits_map_to_statuses(tmp_path):
     for e in range(16):
         want = DEVICE if e & 8 else FORMAT if e & 3 else OVERFLOW if e & 4 else 0
         assert got[e] == want, e
+
+
+def test_drivers_are_built_and_print_usage_without_a_gpu():
+    """The CLI and the dakc.hpp example link against the in-tree library and
+    answer usage requests before any device call."""
+    import subprocess
+    from pathlib import Path
+
+    pkg = Path(__file__).resolve().parents[1] / "dakc_amd"
+    r = subprocess.run([str(pkg / "dakc"), "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage: dakc -f" in r.stdout
+    r = subprocess.run([str(pkg / "dakc")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stdout
+    r = subprocess.run([str(pkg / "hpp_example")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

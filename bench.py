@@ -56,7 +56,7 @@ def measured_traffic(workload: str, kernel: str):
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
-    tag = {"genome": "genome_k31", "iid": "iid_k31"}.get(workload)
+    tag = {"genome": "genome_k31", "genome_k55": "genome_k55", "iid": "iid_k31"}.get(workload)
     data = json.loads(f.read_text()).get(tag, {}).get("kernels", {})
     for name, v in data.items():
         if f"dk::{kernel}<" in name:

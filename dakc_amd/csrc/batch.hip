@@ -77,6 +77,118 @@ static int table_to_host(const MsdBatch &b, size_t j, uint64_t n, uint64_t d, da
   return DAKC_OK;
 }
 
+// Copy a device table (src: keys and counts in device memory, the rest of its
+// fields set; owner-bin offsets in host memory, or binoff_dev on the device)
+// into device memory of its own from the context's table pool, which
+// dakc_table_free gives back (a DevTableHold, as for a served table).  For
+// jobs with table_on_device, whose tables may together outgrow host memory.
+static int copy_words(void *dst, const void *src, size_t bytes, hipStream_t st) {
+  if (!bytes) return DAKC_OK;
+  if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0 && bytes % 4 == 0) {
+    launch_copy_u32((uint32_t *)dst, (const uint32_t *)src, bytes / 4, st);
+    return hipGetLastError() == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+  }
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess ? DAKC_OK : DAKC_E_DEVICE;
+}
+
+// Room for a table's keys (kb bytes) and counts (cb) in the context's table
+// pool: the first arena with room for both, else a new arena of at least
+// 4 GiB.  Freed tables give their ranges back to their arena, which the pool
+// keeps, so the tables of a stream of batches reuse memory instead of
+// allocating it again (large allocations after frees measured up to seconds
+// each on the MI355X).  Arenas with no live table are let go only when a new
+// one cannot be allocated.
+static std::shared_ptr<DevArena> pool_alloc(dakc_ctx *c, size_t kb, size_t cb, void **keys, void **counts) {
+  std::lock_guard<std::mutex> g(c->table_pool_mu);
+  for (auto &a : c->table_pool) {
+    void *k = a->alloc(kb);
+    if (!k) continue;
+    void *n = a->alloc(cb);
+    if (!n) {
+      a->release(k, kb);
+      continue;
+    }
+    *keys = k;
+    *counts = n;
+    return a;
+  }
+  const size_t need = kb + cb;
+  auto a = std::make_shared<DevArena>();
+  if (a->init(c->device, std::max<size_t>(need, (size_t)4 << 30)) && a->init(c->device, need)) {
+    c->table_pool.erase(std::remove_if(c->table_pool.begin(), c->table_pool.end(),
+                                       [](const std::shared_ptr<DevArena> &x) { return x->largest_free() == x->size; }),
+                        c->table_pool.end());
+    if (a->init(c->device, need)) return nullptr;
+  }
+  *keys = a->alloc(kb);
+  *counts = a->alloc(cb);
+  c->table_pool.push_back(a);
+  return a;
+}
+
+static int own_device_table(dakc_ctx *c, hipStream_t st, const dakc_table &src, const uint64_t *binoff_dev,
+                            dakc_table *out) {
+  std::memset(out, 0, sizeof(*out));
+  const size_t kb = std::max<size_t>(((size_t)src.n * src.key_words * 8 + 255) & ~(size_t)255, 256);
+  const size_t cb = std::max<size_t>(((size_t)src.n * 4 + 255) & ~(size_t)255, 256);
+  void *keys = nullptr, *counts = nullptr;
+  auto arena = pool_alloc(c, kb, cb, &keys, &counts);
+  if (!arena) return DAKC_E_NOMEM;
+  auto *h = new (std::nothrow) DevTableHold();
+  uint64_t *binoff = src.num_bins ? (uint64_t *)std::calloc((size_t)src.num_bins + 1, 8) : nullptr;
+  if (!h || (src.num_bins && !binoff)) {
+    delete h;
+    std::free(binoff);
+    arena->release(keys, kb);
+    arena->release(counts, cb);
+    return DAKC_E_NOMEM;
+  }
+  h->arena = arena;
+  h->parts[0] = {keys, kb};
+  h->parts[1] = {counts, cb};
+  h->bytes = kb + cb;
+  // copied by a kernel on the context's stream (after the count that wrote src)
+  const size_t nk = (size_t)src.n * src.key_words * 8, nc = (size_t)src.n * 4;
+  bool ok = copy_words(keys, src.keys, nk, st) == DAKC_OK && copy_words(counts, src.counts, nc, st) == DAKC_OK &&
+            hipStreamSynchronize(st) == hipSuccess;
+  if (ok && binoff) {
+    const size_t ob = ((size_t)src.num_bins + 1) * 8;
+    ok = binoff_dev ? hipMemcpy(binoff, binoff_dev, ob, hipMemcpyDeviceToHost) == hipSuccess
+                    : (src.bin_offsets ? (std::memcpy(binoff, src.bin_offsets, ob), true) : false);
+  }
+  *out = src;
+  out->keys = (uint64_t *)keys;
+  out->counts = (uint32_t *)counts;
+  out->bin_offsets = binoff;
+  out->on_device = 1;
+  out->priv = h;
+  if (!ok) {
+    dakc_table_free(out);
+    return DAKC_E_DEVICE;
+  }
+  return DAKC_OK;
+}
+
+// job j's table out of the arena: host memory, or device memory of its own
+// when the job asks for a device table
+static int table_out(dakc_ctx *c, hipStream_t st, const MsdBatch &b, size_t j, uint64_t n, uint64_t d,
+                     bool on_device, dakc_table *out) {
+  if (!on_device) return table_to_host(b, j, n, d, out);
+  const Plan &p = b.plans[j];
+  const TableLoc &L = b.locs[j];
+  dakc_table src;
+  std::memset(&src, 0, sizeof(src));
+  src.keys = (uint64_t *)L.keys;
+  src.counts = (uint32_t *)L.counts;
+  src.n = d;
+  src.total = n;
+  src.key_words = p.kw;
+  src.k = p.k;
+  src.num_bins = p.owner_digit ? p.num_pes : 0;
+  src.hash_seed = p.seed;
+  return own_device_table(c, st, src, p.owner_digit ? (const uint64_t *)L.binoff : nullptr, out);
+}
+
 // Count the jobs listed in `idx` (all MSD-planned with the same key width,
 // ordered by final-level size) in one multi-job launch on ws; overflowed jobs
 // go to `redo`.
@@ -107,7 +219,7 @@ static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std
       redo.push_back(idx[j]);
       continue;
     }
-    if (!st) st = table_to_host(b, j, n, d, &outs[idx[j]]);
+    if (!st) st = table_out(c, ws.stream, b, j, n, d, jobs[idx[j]].table_on_device != 0, &outs[idx[j]]);
     status[idx[j]] = st;
   }
   return DAKC_OK;
@@ -165,7 +277,15 @@ int batch_count(dakc_ctx *c, const dakc_job *jobs, size_t njobs, dakc_table *out
   for (size_t i : plain) {
     if (std::find(redo.begin(), redo.end(), i) != redo.end()) c->opts.algo = DAKC_ALGO_LSD;
     lsd_job = jobs[i];
-    status[i] = dakc_count(c, &lsd_job, &outs[i]);
+    if (lsd_job.table_on_device) {
+      // the workspace table of dakc_count_device, copied to memory of its own
+      dakc_table ws_table;
+      status[i] = dakc_count_device(c, &lsd_job, &ws_table);
+      if (!status[i]) status[i] = own_device_table(c, c->ws.stream, ws_table, nullptr, &outs[i]);
+      dakc_table_free(&ws_table);
+    } else {
+      status[i] = dakc_count(c, &lsd_job, &outs[i]);
+    }
     c->opts.algo = saved_algo;
   }
   int first = DAKC_OK;

@@ -224,6 +224,10 @@ struct dakc_ctx {
   dk::DevBuf post_a, post_b;  // scratch of the table histogram / filter / query
   dk::DevBuf conv_in, conv_nl, conv_out;  // FASTQ / line input: staged bytes, newlines, records
   dk::HitterBufs hit;
+  // arenas holding the device tables of dakc_count_batch jobs with
+  // table_on_device; kept when their tables are freed, for the next ones
+  std::vector<std::shared_ptr<dk::DevArena>> table_pool;
+  std::mutex table_pool_mu;
 };
 
 namespace dk {

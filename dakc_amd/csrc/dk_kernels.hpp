@@ -94,6 +94,8 @@ inline int fin_cap(int kw, int nt) { return nt * (kw == 1 ? FIN_ITEMS_1 : FIN_IT
 // Launchers (all asynchronous on `st`).
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st);
+// copy of n 32-bit words; dst and src 16-byte aligned
+void launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipStream_t st);
 struct HistSpecs {
   DigitSpec d[MAX_PASSES_DECL];
   int n;

@@ -43,6 +43,14 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
+// n 32-bit words, 16 bytes per thread and step (both pointers 16-byte aligned)
+__global__ void k_copy_u32(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, uint64_t n) {
+  const uint64_t nv = n / 4, stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (uint64_t i = t; i < nv; i += stride)
+    reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+  for (uint64_t i = nv * 4 + t; i < n; i += stride) dst[i] = src[i];
+}
 
 // Sum of the first `below` of NW per-wave values s[0, NW) (NW a power of two
 // <= 64), in every lane: each lane reads one value and a butterfly of NW
@@ -1929,6 +1937,12 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
   uint64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   k_fill_u32<<<(uint32_t)g, 256, 0, st>>>(p, v, n);
+}
+void launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipStream_t st) {
+  if (!n) return;
+  uint64_t g = (n / 4 + 255) / 256;
+  g = g < 1 ? 1 : g > 8192 ? 8192 : g;
+  k_copy_u32<<<(uint32_t)g, 256, 0, st>>>(dst, src, n);
 }
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
   if (!n) return;

@@ -249,6 +249,28 @@ class KmerCounter:
             for t in outs:
                 self._libh.dakc_table_free(C.byref(t))
 
+    def count_batch_device(self, read_sets=None, *, dev_ptrs=None, sizes=None, **over) -> list["DeviceTable"]:
+        """dakc_count_batch with device tables of their own (table_on_device):
+        host read buffers, or device pointers with their sizes.  Each table
+        keeps its device memory until DeviceTable.free()."""
+        if dev_ptrs is not None:
+            keep = []
+            jobs = [self.job(int(p), int(n), on_device=True, table_on_device=True, **over)
+                    for p, n in zip(dev_ptrs, sizes)]
+        else:
+            keep = [_as_bytes(r) for r in read_sets]
+            jobs = [self.job(b.ctypes.data if b.size else 0, int(b.size), table_on_device=True, **over)
+                    for b in keep]
+        arr = (Job * len(jobs))(*jobs)
+        outs = (Table * len(jobs))()
+        st = self._libh.dakc_count_batch(self._ctx, arr, len(jobs), outs)
+        tabs = [DeviceTable(self, outs[i]) for i in range(len(jobs))]
+        if st != _lib.DAKC_OK:
+            for t in tabs:
+                t.free()
+            check(st, "dakc_count_batch")
+        return tabs
+
     # captured batches and the autotuner
     def batch_create(self, read_sets, *, on_device: bool = False, sizes=None, **over) -> "Batch":
         """Stage many read sets for a captured multi-job launch.  read_sets are

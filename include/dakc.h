@@ -69,9 +69,10 @@ typedef struct dakc_job {
   uint32_t num_pes;    /* P; 1 = single PE */
   int32_t owner_bin;   /* -1: all P owner bins; b >= 0: only k-mers owned by PE b */
   int32_t reads_on_device; /* reads is a device pointer (4-byte aligned) */
-  int32_t table_on_device; /* served mode: dakc_wait returns the table in device
-                            * memory owned by the table (keys/counts are device
-                            * pointers; dakc_table_free releases it) */
+  int32_t table_on_device; /* served mode (dakc_wait) and dakc_count_batch: the
+                            * table is returned in device memory owned by the
+                            * table (keys/counts are device pointers;
+                            * dakc_table_free releases it) */
   uint64_t hash_seed;  /* seed of the owner hash; 0 = the reference's
                         * 0x9E3779B97F4A7C15 (kcounter.cpp:52) */
 } dakc_job;
@@ -83,7 +84,8 @@ typedef struct dakc_job {
  * Memory is owned by the library: release with dakc_table_free.  A device
  * table from dakc_count_device lives in the context's workspace (valid until
  * the next count on it; dakc_table_free only forgets it); one from a served
- * job with table_on_device owns its device memory until dakc_table_free. */
+ * or batched job with table_on_device owns its device memory until
+ * dakc_table_free. */
 typedef struct dakc_table {
   uint64_t *keys;
   uint32_t *counts;
@@ -237,7 +239,9 @@ int dakc_count_noalloc(dakc_ctx *ctx, const dakc_job *job, uint64_t *keys_out, u
 
 /* Batched: many independent jobs, one table each (outs[i] for jobs[i]).  Jobs
  * are counted together, one multi-job launch per stage, in chunks that fit the
- * HBM budget. */
+ * HBM budget.  A job with table_on_device gets a device table of its own, so a
+ * caller streaming more read sets than host memory holds tables for can count
+ * them window by window and release each window's tables. */
 int dakc_count_batch(dakc_ctx *ctx, const dakc_job *jobs, size_t njobs, dakc_table *outs);
 
 /* Captured batch: plan and stage the jobs once (host reads are copied to the

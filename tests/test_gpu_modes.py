@@ -31,6 +31,29 @@ def test_batch_matches_plain_and_oracle():
             assert t.total == n and np.array_equal(t.keys, keys) and np.array_equal(t.counts, counts)
 
 
+def test_batch_device_tables_match_oracle():
+    """dakc_count_batch with table_on_device: every job's table lands in device
+    memory of its own (also the job that overflows the MSD path and is
+    recounted on the LSD path, and an owner-bin job), equals the oracle's, and
+    outlives the next batch on the same context."""
+    sets = read_sets(7, 6)
+    sets.append(np.frombuffer((b"A" * 250 + b"\n") * 30_000, np.uint8))  # one k-mer 6.6e6 times
+    with KmerCounter(k=31, read_len=250) as kc:
+        first = kc.count_batch_device(sets)
+        kc.count_batch_device(sets[:2])[0].free()  # reuses the workspace
+        for s, t in zip(sets, first):
+            assert t.t.on_device == 1
+            keys, counts, n, _ = oracle.count(s, 250, 31)
+            h = t.to_host()
+            assert h.total == n and np.array_equal(h.keys, keys) and np.array_equal(h.counts, counts)
+        binned = kc.count_batch_device(sets[:3], num_pes=8, owner_bin=-1)
+        for s, t in zip(sets[:3], binned):
+            keys, counts, n, offs = oracle.count(s, 250, 31, False, 8, -1)
+            h = t.to_host()
+            assert np.array_equal(h.keys, keys) and np.array_equal(h.counts, counts)
+            assert np.array_equal(h.bin_offsets, offs)
+
+
 def test_batch_and_serve_canonical_match_oracle():
     """Canonical jobs, whose final items are sized per quarter of the key
     range, in one batch with a low solo threshold (some jobs get final

@@ -1629,7 +1629,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
   } s_u;
   __shared__ uint32_t s_h[NB3 + 1];
   __shared__ uint32_t s_mixed[NB3 / 32];
-  __shared__ uint32_t s_wave[NT / 64], s_wmin[NT / 64], s_total, s_sb, s_job, s_fail, s_next;
+  __shared__ uint32_t s_wave[NT / 64], s_wmin[NT / 64], s_total, s_sb, s_job, s_fail, s_next, s_dup;
   __shared__ uint64_t s_excl;
   const uint32_t tid = threadIdx.x;
 
@@ -1709,6 +1709,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     }
 
     // split by the next b3 bits into B (unstable, LDS atomics)
+    if (tid == 0) s_dup = 0;  // set below when two keys of the item are equal
     uint32_t dig[ITEMS], rank[ITEMS];
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
@@ -1743,7 +1744,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     // bin that precede it (smaller, or equal and stored before it); a bin of
     // copies of one k-mer keeps its order.  s_map[place] = position in B.
     uint16_t *s_map = s_u.hp;
-    bool bad = false;
+    bool bad = false, dup = false;
   #pragma unroll
     for (int q = 0; q < ITEMS; ++q) {
       const uint32_t i = tid + q * NT;
@@ -1755,25 +1756,49 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
           if (hi - lo <= RANK_MAX) {
             // four independent LDS reads per step; the keys read past the
             // bin's end (inside B's padding at most) are masked out
-            uint32_t r = 0;
+            uint32_t r = 0, e = 0;
             for (uint32_t t0 = lo; t0 < hi; t0 += 4) {
               KeyT y[4];
   #pragma unroll
               for (int j = 0; j < 4; ++j) y[j] = B[t0 + j];
   #pragma unroll
-              for (int j = 0; j < 4; ++j)
-                r += (t0 + j < hi && (key_lt(y[j], x) || (t0 + j < i && key_eq(y[j], x)))) ? 1u : 0u;
+              for (int j = 0; j < 4; ++j) {
+                const bool in = t0 + j < hi, eq = in && key_eq(y[j], x);
+                r += (in && (key_lt(y[j], x) || (t0 + j < i && eq))) ? 1u : 0u;
+                e += eq ? 1u : 0u;
+              }
             }
             place = lo + r;
+            dup |= e > 1;  // x itself is one of them
           } else {
             bad = true;
           }
+        } else {
+          dup |= hi - lo > 1;  // a bin of copies of one k-mer
         }
         s_map[place] = (uint16_t)i;
       }
     }
     if (bad) atomicOr(err, ERR_OVERFLOW);  // recounted on the LSD path
+    if (dup) s_dup = 1;
     __syncthreads();
+
+    if (!s_dup) {
+      // No two keys of the item are equal: each place is a run of one and
+      // its own output index, so thread t writes places t, t + NT, ...
+      // (consecutive lanes, consecutive entries).
+      const uint64_t excl = fin_prefix(J.flags, sb, nsb, mcnt, n_dev, err, &s_excl);
+  #pragma unroll
+      for (int q = 0; q < ITEMS; ++q) {
+        const uint32_t pl = tid + q * NT;
+        if (pl < mcnt) {
+          const uint32_t sp = s_map[pl];
+          out_keys[excl + pl] = B[sp < mcnt ? sp : 0];
+          out_counts[excl + pl] = 1;
+        }
+      }
+      return false;
+    }
 
     // Runs in place order, places tid * ITEMS .. + ITEMS - 1 per thread: a
     // place starts a run (a head) when its key differs from the previous

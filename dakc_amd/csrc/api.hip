@@ -612,11 +612,16 @@ void msd_describe(const MsdBatch &b, size_t j, const uint8_t *dreads, uint8_t *a
   D.out1 = ar + b.off_out1[j];
   D.out2 = ar + b.off_out2[j];
   D.tcounts = (uint32_t *)(ar + b.off_counts[j]);
+  D.tkeys = D.out1;
+  if (j < b.dest_keys.size() && b.dest_keys[j] && b.dest_counts[j]) {  // a table of its own
+    D.tkeys = b.dest_keys[j];
+    D.tcounts = b.dest_counts[j];
+  }
   D.nd = (uint64_t *)(mt + j * META_STRIDE);
   D.err = (uint32_t *)(mt + j * META_STRIDE + 16);
   D.flags = (uint64_t *)(ar + b.off_flags[j]);
   L = TableLoc();
-  L.keys = D.out1;
+  L.keys = D.tkeys;
   L.counts = D.tcounts;
   L.binoff = p.owner_digit ? (uint64_t *)(ar + b.off_binoff[j]) : nullptr;
   L.nd = D.nd;

@@ -189,6 +189,51 @@ static int table_out(dakc_ctx *c, hipStream_t st, const MsdBatch &b, size_t j, u
   return own_device_table(c, st, src, p.owner_digit ? (const uint64_t *)L.binoff : nullptr, out);
 }
 
+// Job j's table, written by the final level into pool memory reserved for it
+// (keys kb, counts cb bytes): the ranges past the table go back to the pool
+// and the table owns the rest.
+static int table_in_place(const MsdBatch &b, size_t j, uint64_t n, uint64_t d, const std::shared_ptr<DevArena> &arena,
+                          size_t kb, size_t cb, dakc_table *out) {
+  const Plan &p = b.plans[j];
+  const TableLoc &L = b.locs[j];
+  void *keys = b.dest_keys[j];
+  uint32_t *counts = b.dest_counts[j];
+  const size_t ku = std::max<size_t>(((size_t)d * p.kw * 8 + 255) & ~(size_t)255, 256);
+  const size_t cu = std::max<size_t>(((size_t)d * 4 + 255) & ~(size_t)255, 256);
+  std::memset(out, 0, sizeof(*out));
+  auto *h = new (std::nothrow) DevTableHold();
+  uint64_t *binoff = p.owner_digit ? (uint64_t *)std::calloc((size_t)p.num_pes + 1, 8) : nullptr;
+  bool ok = h && (!p.owner_digit || binoff);
+  if (ok && p.owner_digit)
+    ok = hipMemcpy(binoff, L.binoff, ((size_t)p.num_pes + 1) * 8, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) {
+    const int st = h && (!p.owner_digit || binoff) ? DAKC_E_DEVICE : DAKC_E_NOMEM;
+    delete h;
+    std::free(binoff);
+    arena->release(keys, kb);
+    arena->release(counts, cb);
+    return st;
+  }
+  if (kb > ku) arena->release((uint8_t *)keys + ku, kb - ku);
+  if (cb > cu) arena->release((uint8_t *)counts + cu, cb - cu);
+  h->arena = arena;
+  h->parts[0] = {keys, ku};
+  h->parts[1] = {counts, cu};
+  h->bytes = ku + cu;
+  out->keys = (uint64_t *)keys;
+  out->counts = counts;
+  out->n = d;
+  out->total = n;
+  out->key_words = p.kw;
+  out->k = p.k;
+  out->num_bins = p.owner_digit ? p.num_pes : 0;
+  out->hash_seed = p.seed;
+  out->bin_offsets = binoff;
+  out->on_device = 1;
+  out->priv = h;
+  return DAKC_OK;
+}
+
 // Count the jobs listed in `idx` (all MSD-planned with the same key width,
 // ordered by final-level size) in one multi-job launch on ws; overflowed jobs
 // go to `redo`.
@@ -205,21 +250,62 @@ static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std
     js.push_back(&jobs[i]);
   }
   msd_layout(b, ps, copy, solo_items(c));
+  // A job with table_on_device has its final level write straight into pool
+  // memory of its own, reserved for the most entries it can have (one per
+  // k-mer slot) and trimmed to the table afterwards; when the pool cannot
+  // give it, the table is written in the arena and copied out.
+  struct Reserved {
+    std::shared_ptr<DevArena> arena;
+    size_t kb = 0, cb = 0;
+  };
+  std::vector<Reserved> res(idx.size());
+  b.dest_keys.assign(idx.size(), nullptr);
+  b.dest_counts.assign(idx.size(), nullptr);
+  for (size_t j = 0; j < idx.size(); ++j) {
+    if (!jobs[idx[j]].table_on_device) continue;
+    const Plan &p = ps[j];
+    res[j].kb = std::max<size_t>(((size_t)p.nmax * p.kw * 8 + 255) & ~(size_t)255, 256);
+    res[j].cb = std::max<size_t>(((size_t)p.nmax * 4 + 255) & ~(size_t)255, 256);
+    void *k = nullptr, *n = nullptr;
+    res[j].arena = pool_alloc(c, res[j].kb, res[j].cb, &k, &n);
+    if (res[j].arena) {
+      b.dest_keys[j] = k;
+      b.dest_counts[j] = (uint32_t *)n;
+    }
+  }
+  auto unreserve = [&](size_t j) {
+    if (!res[j].arena) return;
+    res[j].arena->release(b.dest_keys[j], res[j].kb);
+    res[j].arena->release(b.dest_counts[j], res[j].cb);
+    res[j].arena.reset();
+  };
   int s = msd_stage(ws, b, js);
-  if (s) return s;
-  msd_enqueue(ws, b, false);
-  if (hipStreamSynchronize(ws.stream) != hipSuccess) return DAKC_E_DEVICE;
+  if (!s) {
+    msd_enqueue(ws, b, false);
+    if (hipStreamSynchronize(ws.stream) != hipSuccess) s = DAKC_E_DEVICE;
+  }
   std::vector<uint8_t> meta(b.meta_bytes);
-  if (hipMemcpy(meta.data(), ws.meta.p, b.meta_bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    return DAKC_E_DEVICE;
+  if (!s && hipMemcpy(meta.data(), ws.meta.p, b.meta_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    s = DAKC_E_DEVICE;
+  if (s) {
+    for (size_t j = 0; j < idx.size(); ++j) unreserve(j);
+    return s;
+  }
   for (size_t j = 0; j < idx.size(); ++j) {
     uint64_t n = 0, d = 0;
     int st = msd_finish(b, j, meta.data(), &n, &d);
     if (st == STATUS_MSD_OVERFLOW) {
+      unreserve(j);
       redo.push_back(idx[j]);
       continue;
     }
-    if (!st) st = table_out(c, ws.stream, b, j, n, d, jobs[idx[j]].table_on_device != 0, &outs[idx[j]]);
+    if (!st && res[j].arena) {
+      st = table_in_place(b, j, n, d, res[j].arena, res[j].kb, res[j].cb, &outs[idx[j]]);
+      res[j].arena.reset();  // the table holds it now, or gave it back
+    } else {
+      unreserve(j);
+      if (!st) st = table_out(c, ws.stream, b, j, n, d, jobs[idx[j]].table_on_device != 0, &outs[idx[j]]);
+    }
     status[idx[j]] = st;
   }
   return DAKC_OK;

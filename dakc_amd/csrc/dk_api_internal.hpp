@@ -111,6 +111,9 @@ struct MsdBatch {
   size_t meta_bytes = 0, desc_bytes = 0;
   std::vector<TableLoc> locs;
   std::vector<JobDesc> descs;  // host copies of the job descriptors
+  // where job j's final level writes its table (empty or null: the arena)
+  std::vector<void *> dest_keys;
+  std::vector<uint32_t *> dest_counts;
 };
 
 // Device workspace of one in-flight job (one per served slot).

@@ -171,6 +171,7 @@ struct JobDesc {
   void *out1;         // L1 tiles (tile * tcap1 keys each); later the table keys
   void *out2;         // L2 regions ((b * ngroups + g) * cap2 keys each)
   uint32_t *tcounts;  // table counts
+  void *tkeys;        // table keys: out1, or memory the table owns (batched device tables)
   uint64_t *nd;       // [0] k-mers, [1] distinct
   uint64_t *flags;    // nsb look-back words, zeroed before the launch
   uint32_t *err;      // this job's error bits

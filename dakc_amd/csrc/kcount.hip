@@ -1448,7 +1448,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
   // the bins are contiguous, so a wave's 64 pairs land in a few lines
   const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, J.err, s_excl);
-  uint64_t *out_keys = reinterpret_cast<uint64_t *>(J.out1);
+  uint64_t *out_keys = reinterpret_cast<uint64_t *>(J.tkeys);
 #pragma unroll
   for (int i = 0; i < SPT; ++i)
     if (tid + i * NT < H) {
@@ -1600,7 +1600,7 @@ DK_DEV bool fin_distinct2(const JobDesc &J, const MsdArgs &m, const K2 (&keys)[I
   }
   if (big) atomicOr(J.err, ERR_OVERFLOW);  // recounted on the LSD path
   const uint64_t excl = fin_prefix(J.flags, sb, nsb, H, J.nd, J.err, s_excl);
-  K2 *out_keys = reinterpret_cast<K2 *>(J.out1);
+  K2 *out_keys = reinterpret_cast<K2 *>(J.tkeys);
 #pragma unroll
   for (int i = 0; i < SPT; ++i) {
     const uint32_t p = tid + i * NT;
@@ -1669,7 +1669,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     if (tid == 0) s_fail = (uint32_t)fo > (uint32_t)(fo >> 32) + 32 ? 1u : 0u;
     const MsdArgs m = J.m;
     const KeyT *in2 = reinterpret_cast<const KeyT *>(J.out2);
-    KeyT *out_keys = reinterpret_cast<KeyT *>(J.out1);
+    KeyT *out_keys = reinterpret_cast<KeyT *>(J.tkeys);
     uint32_t *out_counts = J.tcounts;
     uint64_t *n_dev = J.nd;
     uint32_t *err = J.err;

@@ -1306,6 +1306,21 @@ DK_DEV uint32_t fin_bin(const MsdArgs &m, const FinItem &f, const KeyT &x) {
   return ((key_bits(x, m.d2.shift, m.d2.nbits) - f.s0) << m.d3.nbits) | key_bits(x, m.d3.shift, m.d3.nbits);
 }
 
+// Bins of the final level's general path: as many bits below the digit as
+// keep the item's cnt << b3 bins within nb3 (the plan's b3 is sized for its
+// widest items; an item of few digits, as canonical counts make where keys
+// are dense, gets finer bins, and the general path's ranking costs grow with
+// the square of a bin's keys).
+DK_DEV uint32_t fin_gen_b3(const MsdArgs &m, uint32_t cnt, uint32_t nb3) {
+  uint32_t b3 = 0;
+  while (b3 < m.d2.shift && b3 < 11 && (cnt << (b3 + 1)) <= nb3) ++b3;
+  return b3;
+}
+template <typename KeyT>
+DK_DEV uint32_t fin_gen_bin(const MsdArgs &m, uint32_t s0, uint32_t b3, const KeyT &x) {
+  return ((key_bits(x, m.d2.shift, m.d2.nbits) - s0) << b3) | key_bits(x, m.d2.shift - b3, b3);
+}
+
 DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t H, uint64_t *n_dev,
                            uint32_t *err, uint64_t *s_excl) {
   constexpr uint32_t epoch = 1;  // flags are zeroed before every launch
@@ -1708,18 +1723,24 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
         return false;
     }
 
-    // split by the next b3 bits into B (unstable, LDS atomics)
+    // split by the item's own bin bits into B (unstable, LDS atomics); its
+    // bins past the distinct-key path's are cleared first
     if (tid == 0) s_dup = 0;  // set below when two keys of the item are equal
+    const uint32_t gb3 = fin_gen_b3(m, fi.cnt, NB3), gbins = fi.cnt << gb3;
+    if (gbins > nd3) {
+      for (uint32_t i = nd3 + 1 + tid; i <= gbins; i += NT) s_h[i] = 0;
+      __syncthreads();
+    }
     uint32_t dig[ITEMS], rank[ITEMS];
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       if ((uint32_t)u < mine) {
-        dig[u] = fin_bin(m, fi, keys[u]);
+        dig[u] = fin_gen_bin(m, fi.s0, gb3, keys[u]);
         rank[u] = atomicAdd(&s_h[dig[u]], 1u);
       }
     }
     __syncthreads();
-    scan_lds<NT, false>(s_h, nd3, s_wave, &s_total);  // at the register limit
+    scan_lds<NT, false>(s_h, gbins, s_wave, &s_total);  // at the register limit
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
       if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
@@ -1735,7 +1756,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       if (i < mcnt) {
         const KeyT x = B[i];
         keys[q] = x;
-        dig[q] = fin_bin(m, fi, x);
+        dig[q] = fin_gen_bin(m, fi.s0, gb3, x);
         if (!key_eq(B[s_h[dig[q]]], x)) atomicOr(&s_mixed[dig[q] >> 5], 1u << (dig[q] & 31));
       }
     }

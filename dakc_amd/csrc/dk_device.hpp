@@ -65,10 +65,14 @@ DK_DEV uint64_t murmur64a(const K2 &key, uint64_t seed) {
 }
 constexpr uint64_t OWNER_SEED = 0x9E3779B97F4A7C15ULL;  // kcounter.cpp:52, the default seed
 
-// owner PE of a k-mer (kcounter.cpp:50-54 with the job's seed)
+// owner PE of a k-mer (kcounter.cpp:50-54 with the job's seed); a power-of-two
+// P (as the reference's 64-PE runs) takes the low bits, which is the same
+// remainder without the 64-bit division
 template <typename KeyT>
 DK_DEV uint32_t owner_of(const KeyT &x, uint32_t num_pes, uint64_t seed) {
-  return (uint32_t)(murmur64a(x, seed) % (uint64_t)num_pes);
+  const uint64_t h = murmur64a(x, seed);
+  if ((num_pes & (num_pes - 1)) == 0) return (uint32_t)h & (num_pes - 1);
+  return (uint32_t)(h % (uint64_t)num_pes);
 }
 
 // bits [shift, shift + nbits) of the key value, nbits <= 32

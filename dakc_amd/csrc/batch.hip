@@ -98,6 +98,13 @@ static int copy_words(void *dst, const void *src, size_t bytes, hipStream_t st) 
 // allocating it again (large allocations after frees measured up to seconds
 // each on the MI355X).  Arenas with no live table are let go only when a new
 // one cannot be allocated.
+// Give the pool's empty arenas back to the device (caller holds table_pool_mu).
+static void pool_drop_empty_locked(dakc_ctx *c) {
+  c->table_pool.erase(std::remove_if(c->table_pool.begin(), c->table_pool.end(),
+                                     [](const std::shared_ptr<DevArena> &x) { return x->largest_free() == x->size; }),
+                      c->table_pool.end());
+}
+
 static std::shared_ptr<DevArena> pool_alloc(dakc_ctx *c, size_t kb, size_t cb, void **keys, void **counts) {
   std::lock_guard<std::mutex> g(c->table_pool_mu);
   for (auto &a : c->table_pool) {
@@ -115,9 +122,7 @@ static std::shared_ptr<DevArena> pool_alloc(dakc_ctx *c, size_t kb, size_t cb, v
   const size_t need = kb + cb;
   auto a = std::make_shared<DevArena>();
   if (a->init(c->device, std::max<size_t>(need, (size_t)4 << 30)) && a->init(c->device, need)) {
-    c->table_pool.erase(std::remove_if(c->table_pool.begin(), c->table_pool.end(),
-                                       [](const std::shared_ptr<DevArena> &x) { return x->largest_free() == x->size; }),
-                        c->table_pool.end());
+    pool_drop_empty_locked(c);
     if (a->init(c->device, need)) return nullptr;
   }
   *keys = a->alloc(kb);
@@ -280,6 +285,25 @@ static int run_msd_group(dakc_ctx *c, JobWs &ws, const dakc_job *jobs, const std
     res[j].arena.reset();
   };
   int s = msd_stage(ws, b, js);
+  if (s == DAKC_E_NOMEM) {
+    // the reservations may be what left the workspace no room: drop them and
+    // the pool's empty arenas (the tables are then written in the workspace
+    // and copied out) and stage again
+    bool any = false;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      any |= res[j].arena != nullptr;
+      unreserve(j);
+      b.dest_keys[j] = nullptr;
+      b.dest_counts[j] = nullptr;
+    }
+    if (any) {
+      {
+        std::lock_guard<std::mutex> g(c->table_pool_mu);
+        pool_drop_empty_locked(c);
+      }
+      s = msd_stage(ws, b, js);
+    }
+  }
   if (!s) {
     msd_enqueue(ws, b, false);
     if (hipStreamSynchronize(ws.stream) != hipSuccess) s = DAKC_E_DEVICE;

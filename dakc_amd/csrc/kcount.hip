@@ -1343,6 +1343,18 @@ DK_DEV uint64_t fin_prefix(uint64_t *flags, uint32_t sb, uint32_t nsb, uint32_t 
 
 constexpr uint64_t HASH_EMPTY = ~0ull;  // no k-mer of k < 32 bases has all 64 bits set
 
+// Empty hash table of fin_distinct: HT keys of HASH_EMPTY, HT zero counts.
+template <int NT, int HT>
+DK_DEV void fin_clear(uint64_t *lds) {
+  uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);
+  const uint32_t tid = opaque_tid();
+#pragma unroll
+  for (int j = 0; j < HT / NT; ++j) {
+    lds[tid + j * NT] = HASH_EMPTY;
+    tc[tid + j * NT] = 0;
+  }
+}
+
 // Distinct-key path of the final level (keys of k < 32 bases): the gathered
 // keys go into an LDS hash table (equal keys add to one count), the occupied
 // slots are split by the next b3 bits, the few distinct keys of each bin are
@@ -1361,12 +1373,8 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   uint64_t *tk = lds;                                     // HT keys, then the ordered keys
   uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);  // HT counts, then the ordered counts
   uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
-  // *s_fail: the caller's skip decision (set before the keys were loaded)
-#pragma unroll
-  for (int j = 0; j < SPT; ++j) {
-    tk[tid + j * NT] = HASH_EMPTY;
-    tc[tid + j * NT] = 0;
-  }
+  // *s_fail: the caller's skip decision (set before the keys were loaded);
+  // the caller has cleared the table (fin_clear) while the keys loaded
   __syncthreads();
   if (*s_fail) return false;
   // first probes of all keys back to back; collisions walk on from there
@@ -1703,6 +1711,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
         in2, r0, r1, m.ngroups, (uint64_t)b * m.ngroups * m.cap2, m.cap2, s_u.seg, s_wave, &s_total,
         keys, mine);
     if (tot > (uint32_t)CAP && fi.cnt > 1) return true;
+    // the distinct-key path's table is cleared while the keys load (it
+    // shares no LDS with the gather)
+    if constexpr (sizeof(KeyT) == 8)
+      if (J.rd.k < 32) fin_clear<NT, HT>(reinterpret_cast<uint64_t *>(B));
     between();
     if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
     const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;

@@ -1047,23 +1047,30 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
         l1_prefetch_raw<NT, RAWW>(rdn, nitem - cur.lo, s_raw);
       }
     });
-    if (m.d1.num_pes <= 1) {  // first-level digit: the top bits alone
+    // digit << 16 | rank of each slot; the digit is the top bits alone, or
+    // owner * 2^b1 + top bits.  The rank is joined to the digit after the
+    // branch, so the atomics of a thread stay in flight together (joined
+    // inside, every atomic waited for its result before the next went out).
+    uint32_t dg[ITEMS], rk[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        if ((vmask >> j) & 1) {
-          const uint32_t d = key_bits(keys[j], m.d1.shift, m.d1.nbits);
-          dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
-        }
-      }
-    } else {  // owner * 2^b1 + top bits
+    for (int j = 0; j < ITEMS; ++j) dg[j] = rk[j] = 0;
+    if (m.d1.num_pes <= 1) {
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
+      for (int j = 0; j < ITEMS; ++j)
         if ((vmask >> j) & 1) {
-          const uint32_t d = digit_eval(m.d1, keys[j], rd.seed);
-          dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+          dg[j] = key_bits(keys[j], m.d1.shift, m.d1.nbits);
+          rk[j] = atomicAdd(&s_h[dg[j]], 1u);
         }
-      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+        if ((vmask >> j) & 1) {
+          dg[j] = digit_eval(m.d1, keys[j], rd.seed);
+          rk[j] = atomicAdd(&s_h[dg[j]], 1u);
+        }
     }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) dr[j] = (dg[j] << 16) | rk[j];
     __syncthreads();
     scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
     const uint32_t total = s_h[m.nb1];
@@ -1075,9 +1082,13 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
     if (tid == 0 && m.nb1 >= NT) off[(uint64_t)NT * m.ntiles1] = (uint16_t)s_h[NT];
     if (tid == 0 && total)
       atomicAdd(reinterpret_cast<unsigned long long *>(n_dev), (unsigned long long)total);
+    // places first (every slot reads its digit's start, an empty one s_h[0]),
+    // then the writes: a read inside the write's branch waited for each
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) dr[j] = s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-      if ((vmask >> j) & 1) sm.stage[s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = keys[j];
+      if ((vmask >> j) & 1) sm.stage[dr[j]] = keys[j];
     __syncthreads();
     write_staged<KeyT, NT, ITEMS>(sm.stage, out1 + (uint64_t)tile * m.tcap1, total, tid);
   }
@@ -1114,13 +1125,19 @@ __global__ __launch_bounds__(NT) void k_l1_keys(JobDesc one) {
   s_h[tid] = 0;
   if (tid == 0) s_h[1024] = 0;
   __syncthreads();
-  uint32_t dr[ITEMS];  // digit << 16 | rank within the digit
+  // digit << 16 | rank within the digit, joined after the branch so the
+  // atomics stay in flight together
+  uint32_t dr[ITEMS], rk[ITEMS];
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j)
+  for (int j = 0; j < ITEMS; ++j) {
+    dr[j] = rk[j] = 0;
     if ((vmask >> j) & 1) {
-      const uint32_t d = digit_eval(m.d1, keys[j], rd.seed);
-      dr[j] = (d << 16) | atomicAdd(&s_h[d], 1u);
+      dr[j] = digit_eval(m.d1, keys[j], rd.seed);
+      rk[j] = atomicAdd(&s_h[dr[j]], 1u);
     }
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) dr[j] = (dr[j] << 16) | rk[j];
   __syncthreads();
   scan_lds<NT>(s_h, m.nb1, s_wave, &s_total);
   const uint32_t total = s_h[m.nb1];
@@ -1129,8 +1146,10 @@ __global__ __launch_bounds__(NT) void k_l1_keys(JobDesc one) {
   if (tid == 0 && m.nb1 >= NT) off[(uint64_t)NT * m.ntiles1] = (uint16_t)s_h[NT];
   if (tid == 0 && total) atomicAdd(reinterpret_cast<unsigned long long *>(one.nd), (unsigned long long)total);
 #pragma unroll
+  for (int j = 0; j < ITEMS; ++j) dr[j] = s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu);  // places, then the writes
+#pragma unroll
   for (int j = 0; j < ITEMS; ++j)
-    if ((vmask >> j) & 1) stage[s_h[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = keys[j];
+    if ((vmask >> j) & 1) stage[dr[j]] = keys[j];
   __syncthreads();
   write_staged<KeyT, NT, ITEMS>(stage, reinterpret_cast<KeyT *>(one.out1) + (uint64_t)tile * m.tcap1, total, tid);
 }
@@ -1214,9 +1233,13 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     s_h[tid] = 0;  // the previous item's split is done (its stage write-out follows a barrier)
     if (tid == 0) s_h[1024] = 0;
     __syncthreads();
+    // (the join inside the branch makes each atomic wait for its result
+    // before the next goes out; the deferred join of k_l1 measured 1-2%
+    // slower here, where the ranking overlaps the next item's gather)
     uint32_t dr[ITEMS];  // digit << 16 | rank within the digit (< 2^14 keys per item)
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
+      dr[u] = 0;
       if ((uint32_t)u < mine) {
         const uint32_t d = key_bits(v[u], m.d2.shift, m.d2.nbits);  // levels 2 and 3 split by key bits
         dr[u] = (d << 16) | atomicAdd(&s_h[d], 1u);
@@ -1236,8 +1259,10 @@ __global__ __launch_bounds__(NT) void k_l2(BatchArgs a, JobDesc one) {
     if (tid <= m.nd2) off[tid] = (uint16_t)(tot ? s_h[tid] : 0);
     if (tid == 0 && m.nd2 >= NT) off[NT] = (uint16_t)(tot ? s_h[NT] : 0);
 #pragma unroll
+    for (int u = 0; u < ITEMS; ++u) dr[u] = s_h[dr[u] >> 16] + (dr[u] & 0xFFFFu);  // places, then the writes
+#pragma unroll
     for (int u = 0; u < ITEMS; ++u)
-      if ((uint32_t)u < mine) stage[s_h[dr[u] >> 16] + (dr[u] & 0xFFFFu)] = v[u];
+      if ((uint32_t)u < mine) stage[dr[u]] = v[u];
     __syncthreads();
     // the next item's keys have arrived: settle them before this item's
     // stores, so the next iteration waits for nothing
@@ -1419,6 +1444,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     for (int j = 0; j < SPT; ++j) {
       ek[j] = tk[tid + j * NT];
       ec[j] = tc[tid + j * NT];
+      ed[j] = er[j] = 0;
       if (ek[j] != HASH_EMPTY) {
         ed[j] = fin_bin(m, fi, ek[j]);
         er[j] = atomicAdd(&s_h[ed[j]], 1u);
@@ -1426,12 +1452,15 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
     }
     __syncthreads();
     scan_lds<NT, false>(s_h, nd3, s_wave, s_total);  // at the register limit
+    // places first (an empty slot reads s_h[0]), then the writes: a read
+    // inside the write's branch waited for each
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) er[j] += s_h[ed[j]];
 #pragma unroll
     for (int j = 0; j < SPT; ++j)
       if (ek[j] != HASH_EMPTY) {
-        const uint32_t p = s_h[ed[j]] + er[j];
-        tk[p] = ek[j];
-        tc[p] = ec[j];
+        tk[er[j]] = ek[j];
+        tc[er[j]] = ec[j];
       }
   }
   const uint32_t H = s_h[nd3];
@@ -1440,17 +1469,26 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
   __syncthreads();
   // order the distinct keys of each bin, positions tid, tid + NT, ... < H:
   // place = bin start + number of smaller keys in the bin
+  // (the pairs, then their bins' bounds, each group of reads waiting once;
+  // positions at or past H read table slots and bin 0, and are not used)
   uint64_t x[SPT];
-  uint32_t xc[SPT], place[SPT];
+  uint32_t xc[SPT], place[SPT], bnd[SPT];
   bool big = false;
+#pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    x[i] = tk[tid + i * NT];
+    xc[i] = tc[tid + i * NT];
+  }
+#pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    const uint32_t d = tid + i * NT < H ? fin_bin(m, fi, x[i]) : 0u;
+    bnd[i] = s_h[d] | (s_h[d + 1] << 16);  // bin starts are below HT <= 2^16
+  }
 #pragma unroll
   for (int i = 0; i < SPT; ++i) {
     const uint32_t p = tid + i * NT;
     if (p < H) {
-      x[i] = tk[p];
-      xc[i] = tc[p];
-      const uint32_t d = fin_bin(m, fi, x[i]);
-      const uint32_t lo = s_h[d], hi = s_h[d + 1];
+      const uint32_t lo = bnd[i] & 0xFFFFu, hi = bnd[i] >> 16;
       uint32_t r = 0;
       if (hi - lo > 1) {
         if (hi - lo > 256) {
@@ -1746,6 +1784,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     uint32_t dig[ITEMS], rank[ITEMS];
   #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
+      dig[u] = rank[u] = 0;
       if ((uint32_t)u < mine) {
         dig[u] = fin_gen_bin(m, fi.s0, gb3, keys[u]);
         rank[u] = atomicAdd(&s_h[dig[u]], 1u);
@@ -1754,8 +1793,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     __syncthreads();
     scan_lds<NT, false>(s_h, gbins, s_wave, &s_total);  // at the register limit
   #pragma unroll
+    for (int u = 0; u < ITEMS; ++u) rank[u] += s_h[dig[u]];  // places, then the writes
+  #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
-      if ((uint32_t)u < mine) B[s_h[dig[u]] + rank[u]] = keys[u];
+      if ((uint32_t)u < mine) B[rank[u]] = keys[u];
     __syncthreads();
 
     // Bins whose keys are not all equal (mixed) are ranked below; a mixed bin

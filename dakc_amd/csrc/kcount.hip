@@ -961,6 +961,38 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
   const uint64_t kbits = k >= 64 ? ~0ULL : ((1ULL << k) - 1);  // one bit per base
   const uint64_t kmask = sizeof(KeyT) == 8 ? (k >= 32 ? ~0ULL : ((1ULL << (2 * k)) - 1))
                                            : (k >= 64 ? ~0ULL : ((1ULL << (2 * k - 64)) - 1));
+  // Which windows are k-mers, for all of them at once.  Window j (position
+  // p0 + j) must be one of this thread's slots and end at or before the
+  // record's first 'M' (p0 + j + k <= mp): j < nl.  None of its k bases may
+  // be invalid: the ITEMS windows span ITEMS - 1 + k positions from p0 (the
+  // read's end bounds the windows that count) and reads seldom hold an
+  // invalid base, so one test clears every window and only a thread whose
+  // span holds one tests them one by one.
+  const int lim = (int)mp - k - (int)p0 + 1;
+  const uint32_t nl = lim <= 0 ? 0u : ((uint32_t)lim < nslots ? (uint32_t)lim : nslots);
+  uint32_t ok = (1u << nl) - 1;  // nl <= ITEMS < 32
+  const uint32_t span = min((uint32_t)(ITEMS - 1 + k), rd.L - p0);  // <= 46 one-word, <= 71 two-word
+  const bool anybad = span < 64 ? (vlo & ((1ULL << span) - 1)) != 0
+                                : (vlo != 0 || (vhi & ((1ULL << (span - 64)) - 1)) != 0);
+  if (anybad) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint64_t win = j ? (vlo >> j) | (vhi << (64 - j)) : vlo;
+      if ((win & kbits) != 0) ok &= ~(1u << j);
+    }
+  }
+  // kcounter_funcs.cpp:97: the run left open at the end of a record is
+  // emitted only if it starts before READLEN-1, so a lone base in the last
+  // position never forms a 1-mer
+  if (k == 1) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = p0 + j;
+      if (((ok >> j) & 1) && p == rd.L - 1 &&
+          (rd.L < 2 || ((sm.inv[rb + ((p - 1) >> 5)] >> ((p - 1) & 31)) & 1)))
+        ok &= ~(1u << j);
+    }
+  }
   KeyT fw = window_key<KeyT>(sm.code[w0], sm.code[w0 + 1], sm.code[w0 + 2], 2 * sh0, k);
   KeyT rc = rd.canonical ? revcomp(fw, k) : fw;
 #pragma unroll
@@ -970,21 +1002,14 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
       key_push(fw, b, kmask);
       if (rd.canonical) rc_push(rc, b, k);
     }
-    const uint64_t win = j ? (vlo >> j) | (vhi << (64 - j)) : vlo;
-    const uint32_t p = p0 + j;
-    bool v = (uint32_t)j < nslots && (win & kbits) == 0 && p + (uint32_t)k <= mp;
-    // kcounter_funcs.cpp:97: the run left open at the end of a record is
-    // emitted only if it starts before READLEN-1, so a lone base in the last
-    // position never forms a 1-mer
-    if (k == 1 && v && p == rd.L - 1 &&
-        (rd.L < 2 || ((sm.inv[rb + ((p - 1) >> 5)] >> ((p - 1) & 31)) & 1)))
-      v = false;
-    KeyT key = rd.canonical ? (key_lt(rc, fw) ? rc : fw) : fw;
-    if (v && rd.num_pes > 1 && rd.owner_bin >= 0 && owner_of(key, rd.num_pes, rd.seed) != (uint32_t)rd.owner_bin)
-      v = false;
-    keys[j] = key;
-    vmask |= (v ? 1u : 0u) << j;
+    keys[j] = rd.canonical ? (key_lt(rc, fw) ? rc : fw) : fw;
   }
+  if (rd.num_pes > 1 && rd.owner_bin >= 0) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+      if (((ok >> j) & 1) && owner_of(keys[j], rd.num_pes, rd.seed) != (uint32_t)rd.owner_bin) ok &= ~(1u << j);
+  }
+  vmask = ok;
 }
 
 // L1: tiles of records -> their k-mers in first-level bucket order, each

@@ -99,24 +99,24 @@ __host__ __device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
 // their 2-bit codes, first base in the top bits, 0 for a base that is not
 // ACGT or lies past the read; inv4 has bit i set for base i not ACGT (or
 // past the read); mmask has 0x80 in the bytes that are 'M' or 'm'.  The same
-// as the byte-at-a-time decode: u = b & 0xDF (upper case), code from ASCII
-// bits 1-2 (A0 C1 T2 G3, then C0 A1 T2 G3), valid iff u is the letter that
-// code belongs to.
+// as the byte-at-a-time decode: u = b & 0xDF (upper case), tt = ASCII bits
+// 1-2 (A0 C1 T2 G3), valid iff u is the letter of tt, code C0 A1 T2 G3.
+// The letter and the code of each byte are 4-entry byte lookups by tt
+// (v_perm_b32 picks byte tt of its second operand): full-rate instructions
+// where the arithmetic form took two quarter-rate multiplies.
 struct Dec4 {
   uint32_t code8, inv4, mmask;
 };
-__host__ __device__ __forceinline__ Dec4 decode4(uint32_t w, uint32_t nv) {
+DK_DEV Dec4 decode4(uint32_t w, uint32_t nv) {
   const uint32_t pm = nv >= 4 ? 0x80808080u : ((1u << (8 * nv)) - 1u) & 0x80808080u;
   const uint32_t u = w & 0xDFDFDFDFu;
   const uint32_t tt = (w >> 1) & 0x03030303u;
-  const uint32_t t0 = tt & 0x01010101u, t1 = (tt >> 1) & 0x01010101u, t01 = t0 & t1;
-  // the letter of each byte's code: A 0x41, C 0x43, T 0x54, G 0x47
-  const uint32_t letter = 0x41414141u + (t0 << 1) + t1 * 0x13u - t01 * 0x0Fu;
+  const uint32_t letter = __builtin_amdgcn_perm(0u, 0x47544341u, tt);  // G T C A for tt 3 2 1 0
   const uint32_t okm = zero_bytes(u ^ letter) & pm;
-  const uint32_t c = (tt ^ t1 ^ 0x01010101u) & ((okm >> 7) * 3u);
+  const uint32_t c = __builtin_amdgcn_perm(0u, 0x03020001u, tt) & ((okm >> 7) * 3u);  // A->1, C->0
   Dec4 d;
-  d.code8 = (c * 0x40100401u) >> 24;                               // fields at bits 0, 8, 16, 24
-  d.inv4 = ~(((okm >> 7) * 0x00204081u) >> 21) & 0xFu;             // bits 7, 15, 23, 31 gathered
+  d.code8 = (c * 0x40100401u) >> 24;                    // fields at bits 0, 8, 16, 24
+  d.inv4 = ~(((okm >> 7) * 0x00204081u) >> 21) & 0xFu;  // bits 7, 15, 23, 31 gathered
   d.mmask = zero_bytes(u ^ 0x4D4D4D4Du) & pm;
   return d;
 }

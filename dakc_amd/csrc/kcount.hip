@@ -607,8 +607,9 @@ struct GatherSmem {
 // consecutive positions, mostly from one segment.  A position's segment is
 // the number of segment starts at or before it, from a bitmask of the starts
 // and per-word prefix counts (no search).  Returns the total; when it exceeds
-// CAP no key is loaded and mine = 0.  Offsets must fit 32 bits
-// (q * stride + row0[q] < 2^32).
+// CAP mine = 0.  Keys past this thread's mine are left as they were (any
+// value, not used).  Offsets must fit 32 bits (q * stride + row0[q] <
+// 2^32).
 // Segment bounds of thread t: row0[q], row1[q] for q = t * TPT + u (0 past nseg).
 template <int TPT>
 DK_DEV void load_rows(const uint16_t *row0, const uint16_t *row1, uint32_t nseg, uint32_t (&r0)[TPT],
@@ -687,8 +688,6 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], cons
   __syncthreads();
   mine = fits && tot > tid ? (tot - tid + NT - 1) / NT : 0;
   if (mine > (uint32_t)ITEMS) mine = ITEMS;
-#pragma unroll
-  for (int u = 0; u < ITEMS; ++u) v[u] = KeyT{};
   // in chunks of CH keys: the word reads, then the segment-offset reads, then
   // the key loads, so each group of LDS reads waits once and the loads of a
   // chunk are in flight while the next chunk's offsets are found
@@ -713,6 +712,9 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], cons
       wp[i] = g.delta[seg] + p;
     }
     __builtin_amdgcn_sched_barrier(0);
+    // a slot past this thread's keys loads nothing and is left as it was
+    // (zeroing them first made the compiler copy every later key register
+    // at each load's branch)
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int u = c0 + i;

@@ -1419,15 +1419,16 @@ template <int NT, int ITEMS, int HT>
 DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&keys)[ITEMS],
                          uint32_t mine, uint64_t *lds, uint32_t *s_h, uint32_t *s_wave,
                          uint32_t *s_total, uint32_t *s_fail, uint64_t *s_excl, uint32_t sb,
-                         uint32_t nsb, const FinItem &fi) {
+                         uint32_t nsb, const FinItem &fi, bool synced) {
   constexpr int SPT = HT / NT;  // table slots per thread
   const uint32_t tid = opaque_tid();
   uint64_t *tk = lds;                                     // HT keys, then the ordered keys
   uint32_t *tc = reinterpret_cast<uint32_t *>(lds + HT);  // HT counts, then the ordered counts
   uint32_t *hc = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(J.nd) + 24);  // fail, ok
   // *s_fail: the caller's skip decision (set before the keys were loaded);
-  // the caller has cleared the table (fin_clear) while the keys loaded
-  __syncthreads();
+  // the caller has cleared the table (fin_clear) while the keys loaded, and
+  // passed a barrier since when `synced`
+  if (!synced) __syncthreads();
   if (*s_fail) return false;
   // first probes of all keys back to back; collisions walk on from there
   uint32_t hh[ITEMS];
@@ -1746,7 +1747,8 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
 
   // One final item fi of job J, its group rows in r0/r1 and the job's
   // overflow counters in fo (tid 0).  `between` runs right after the gather,
-  // when the rows are free again; every thread returns together.  Returns
+  // when the rows are free again, and returns whether it passed a barrier
+  // (uniform); every thread returns together.  Returns
   // true, before `between`, when the item holds more keys than LDS and more
   // than one digit: the caller then runs its digits one by one.
   auto run = [&](const JobDesc &J, const FinItem &fi, uint32_t (&r0)[TPT], uint32_t (&r1)[TPT], uint64_t fo,
@@ -1780,7 +1782,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     // shares no LDS with the gather)
     if constexpr (sizeof(KeyT) == 8)
       if (J.rd.k < 32) fin_clear<NT, HT>(reinterpret_cast<uint64_t *>(B));
-    between();
+    const bool synced = between();
     if (tot > (uint32_t)CAP && tid == 0) atomicOr(err, ERR_OVERFLOW);
     const uint32_t mcnt = tot > (uint32_t)CAP ? 0 : tot;
     // the item's inner digits add nothing to the prefix
@@ -1790,7 +1792,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
       if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
                                                          s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
-                                                         nsb, fi))
+                                                         nsb, fi, synced))
         return false;
     } else {
       static_assert((size_t)HT * 22 <= sizeof(B) && HT % NT == 0 && HT <= 65536,
@@ -2013,7 +2015,9 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       uint32_t d = 0;
       for (;;) {
         const bool split = run(one, cur, r0, r1, fo, [&] {
-          if (!digits) fetch_next();
+          if (digits) return false;
+          fetch_next();  // its barrier follows the table clear
+          return true;
         });
         if (split) digits = true;
         else if (digits) ++d;
@@ -2045,7 +2049,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     bool digits = false;  // more keys than LDS holds: then digit by digit
     uint32_t d = 0;
     for (;;) {
-      const bool split = run(J, cur, r0, r1, fo, [] {});
+      const bool split = run(J, cur, r0, r1, fo, [] { return false; });
       if (split) digits = true;
       else if (digits) ++d;
       if (!digits || d == f.cnt) break;

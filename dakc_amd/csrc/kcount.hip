@@ -559,11 +559,15 @@ __global__ __launch_bounds__(256) void k_bin_offsets(const KeyT *keys, const uin
 // bucket into LDS, sorts it there, run-length reduces it and writes its
 // (kmer, count) entries at the position the decoupled look-back gives.
 
-// exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = total (SHFL as
-// in scan_block)
+// exclusive scan in place of n <= 4*NT u32 in LDS; s[n] = *s_total = total.
+// SHFL: the earlier waves' totals are added by shuffles (one LDS read);
+// otherwise by a loop of LDS reads, which needs fewer registers in a kernel
+// at its register limit.  Every thread adds up the wave totals itself, so
+// one barrier publishes them and a second ends the scan.
 template <int NT, bool SHFL = true>
 DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_total) {
-  const uint32_t tid = opaque_tid();
+  constexpr uint32_t NW = NT / 64;
+  const uint32_t tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   uint32_t v[4], sum = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -571,14 +575,37 @@ DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_tota
     v[q] = i < n ? s[i] : 0;
     sum += v[q];
   }
-  uint32_t pre = scan_block<NT, SHFL>(sum, s_wave, s_total);
+  uint32_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  if constexpr (SHFL) {
+    pre = sum_waves_below<NW>(s_wave, wave, lane);
+    tot = sum_waves_below<NW>(s_wave, NW, lane);
+  } else {
+#pragma nounroll
+    for (uint32_t w = 0; w < NW; ++w) {  // one read in flight: no registers held for all
+      const uint32_t sw = s_wave[w];
+      if (w < wave) pre += sw;
+      tot += sw;
+    }
+  }
+  pre += x - sum;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const uint32_t i = tid * 4 + q;
     if (i < n) s[i] = pre;
     pre += v[q];
   }
-  if (tid == 0) s[n] = *s_total;
+  if (tid == 0) {
+    s[n] = tot;
+    *s_total = tot;
+  }
   __syncthreads();
 }
 
@@ -1764,9 +1791,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     uint64_t *n_dev = J.nd;
     uint32_t *err = J.err;
     const uint32_t nd3 = fi.nbins;
+    // cleared bins and mixed flags: the gather's barriers order these
+    // stores before their first use (the split, after the inserts)
     for (uint32_t i = tid; i <= nd3; i += NT) s_h[i] = 0;
     for (uint32_t w = tid; w < NB3 / 32; w += NT) s_mixed[w] = 0;
-    __syncthreads();
     // look-back slots are digits: this item's count goes to its last digit
     const uint32_t nsb = m.nb1 * m.nd2;
     const uint32_t sb = fi.b * m.nd2 + fi.s0 + fi.cnt - 1;

@@ -1772,6 +1772,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
     }
   };
 
+  // Set by run when its item ended on the one-word distinct-key path, which
+  // reads no LDS after its last barrier (it writes its pairs from registers).
+  bool regs_out = false;
+
   // One final item fi of job J, its group rows in r0/r1 and the job's
   // overflow counters in fo (tid 0).  `between` runs right after the gather,
   // when the rows are free again, and returns whether it passed a barrier
@@ -1820,8 +1824,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       static_assert((size_t)HT * 12 <= sizeof(B) && HT % NT == 0, "hash table does not fit the key buffer");
       if (J.rd.k < 32 && fin_distinct<NT, ITEMS, HT>(J, m, keys, mine, reinterpret_cast<uint64_t *>(B),
                                                          s_h, s_wave, &s_total, &s_fail, &s_excl, sb,
-                                                         nsb, fi, synced))
+                                                         nsb, fi, synced)) {
+        regs_out = true;
         return false;
+      }
     } else {
       static_assert((size_t)HT * 22 <= sizeof(B) && HT % NT == 0 && HT <= 65536,
                     "hash table does not fit the key buffer");
@@ -2041,6 +2047,7 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
       FinItem cur = f;
       bool digits = false;
       uint32_t d = 0;
+      regs_out = false;
       for (;;) {
         const bool split = run(one, cur, r0, r1, fo, [&] {
           if (digits) return false;
@@ -2058,7 +2065,10 @@ __global__ __launch_bounds__(NT, 4) void k_fin(BatchArgs a, JobDesc one) {
         cur = FinItem{f.b, f.s0 + d, 1, 1u << one.m.d3.nbits};
         rows_of(one.m, cur, r0, r1);
       }
-      __syncthreads();  // this item's LDS is free, s_next read by all
+      // this item's LDS is free (s_next was read after fetch_next's barrier):
+      // an item that wrote its pairs from registers read none since its
+      // look-back barrier, so the next item's first stores need no barrier
+      if (digits || !regs_out) __syncthreads();
       t = tn;
       fo = fon;
     }

@@ -900,24 +900,24 @@ DK_DEV void rc_push(K2 &x, uint32_t b, int k) {  // 33 <= k <= 64: the new base 
   x.hi = (x.hi >> 2) | ((uint64_t)(b ^ 3u) << (2 * k - 66));
 }
 
-// LDS of an L1 tile's decoded records: packed 2-bit codes, masks of positions
-// that cannot be in a k-mer, and each record's first 'M'.
+// LDS of an L1 tile's decoded records: packed 2-bit codes and masks of
+// positions that cannot be in a k-mer.
 struct L1Code {
   uint64_t code[L1_CODEW];  // MSB-first 2-bit codes, 32 bases per word
   uint32_t inv[L1_CODEW];   // LSB-first: bit q set = base 32w+q cannot be in a k-mer
-  uint32_t mpos[L1_MAXRT];  // first 'M' (end of read) per record, L if none
 };
 
 // Decode the tile's raw records (in `raw`, records r0 .. r0 + nr - 1 of the
-// job) into 2-bit codes, invalid-base masks and first-'M' positions in LDS,
-// then build this thread's ITEMS windows.  Restates kcounter_funcs.cpp:40-111:
-// a window is a k-mer iff its k bases are all ACGT (0xFF splits runs) and it
-// starts before the record's first 'M' (0xF0 ends the read).  The caller has
-// set mpos to L and made raw and mpos visible (barrier); after the decode's
-// closing barrier `after_decode` runs, and raw is no longer read.
+// job) into 2-bit codes and invalid-base masks in LDS and each record's first
+// 'M' into mpos, then build this thread's ITEMS windows.  Restates
+// kcounter_funcs.cpp:40-111: a window is a k-mer iff its k bases are all ACGT
+// (0xFF splits runs) and it starts before the record's first 'M' (0xF0 ends
+// the read).  The caller has set mpos[r] to L and made raw and mpos visible
+// (barrier); after the decode's closing barrier `after_decode` runs, and raw
+// is no longer read.
 template <typename KeyT, int NT, int ITEMS, typename AfterDecode>
 DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32, L1Code &sm,
-                       KeyT (&keys)[ITEMS], uint32_t &vmask, uint32_t *err,
+                       uint32_t *mpos, KeyT (&keys)[ITEMS], uint32_t &vmask, uint32_t *err,
                        AfterDecode after_decode) {
   const uint32_t tid = opaque_tid();
   const L1Span t = l1_span(rd, tile);
@@ -955,7 +955,7 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
         inv |= d.inv4 << (4 * q);
         if (d.mmask && mloc == 0xFFFFFFFFu) mloc = j0 + 4 * q + (__builtin_ctz(d.mmask) >> 3);
       }
-      if (mloc != 0xFFFFFFFFu) atomicMin(&sm.mpos[r], mloc);
+      if (mloc != 0xFFFFFFFFu) atomicMin(&mpos[r], mloc);
       if (w == 0 && rec[0] == 0) atomicOr(err, ERR_NUL_RECORD);
       if (w == wpl - 1 && b0 + (uint64_t)(r + 1) * rd.stride <= rd.nbytes && rec[rd.L] != '\n')
         atomicOr(err, ERR_SEPARATOR);
@@ -976,7 +976,7 @@ DK_DEV void l1_windows(const ReadsDesc &rd, uint32_t tile, const uint32_t *raw32
   if (r >= nr) return;
   const uint32_t nslots = rd.S - p0 < (uint32_t)ITEMS ? rd.S - p0 : (uint32_t)ITEMS;
   const uint32_t rb = r * rd.wpr, w0 = rb + (p0 >> 5), sh0 = p0 & 31;
-  const uint32_t mp = sm.mpos[r];
+  const uint32_t mp = mpos[r];
   // invalid-base bits from p0 on: vlo = bits [p0, p0 + 64), vhi = the next 32
   const uint64_t i01 = (uint64_t)sm.inv[w0] | ((uint64_t)sm.inv[w0 + 1] << 32);
   uint64_t i23 = sm.inv[w0 + 2];
@@ -1059,6 +1059,7 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   __shared__ Smem sm;
   __shared__ uint32_t s_raw[RAWW];
   __shared__ uint32_t s_h[1024 + 1];
+  __shared__ uint32_t s_mpos[L1_MAXRT];  // first 'M' per record (outside the stage, set a tile ahead)
   __shared__ uint32_t s_wave[NT / 64], s_total;
   __shared__ uint32_t s_job;
   static_assert(NT >= 1024 && L1_MAXRT <= NT, "one histogram slot and record per thread");
@@ -1069,10 +1070,20 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
   const uint32_t hi = lo + per + (x < rem ? 1u : 0u);
   uint32_t item = lo + (blockIdx.x >> 3);
   JobCursor cur;  // the job of the tile being loaded next
+  // A tile's LDS set-up is done during the tile before it: its records
+  // stream in (and its job's last word is assembled) once the previous
+  // records are decoded, its digit counts are cleared and its record ends
+  // set to the read length after the previous tile's staging.  One barrier
+  // at the top of the tile then orders all of it before the decode.
+  const uint32_t tid0 = opaque_tid();
   if (item < hi) {
     if (!ONE) cursor_seek(cur, a.pre1, a.njobs, item, &s_job);
     const ReadsDesc rdf = ONE ? one.rd : job_desc(a, cur.job).rd;
     l1_prefetch_raw<NT, RAWW>(rdf, item - cur.lo, s_raw);
+    l1_patch_tail(rdf, item - cur.lo, s_raw);
+    s_h[tid0] = 0;
+    if (tid0 == 0) s_h[1024] = 0;
+    if (tid0 < L1_MAXRT) s_mpos[tid0] = rdf.L;
   }
   __builtin_amdgcn_s_waitcnt(VMCNT0);
   for (; item < hi; item += nx) {
@@ -1084,21 +1095,19 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
     uint32_t *err = J.err;
     const uint32_t tile = item - cur.lo;
     const uint32_t tid = opaque_tid();
-    __syncthreads();  // the previous tile is written out; this tile's records are in LDS
-    s_h[tid] = 0;
-    if (tid == 0) s_h[1024] = 0;
-    if (tid < rd.RT) sm.ex.mpos[tid] = rd.L;
-    l1_patch_tail(rd, tile, s_raw);
-    __syncthreads();
+    __syncthreads();  // the previous tile is written out; this tile's records and set-up are in LDS
     KeyT keys[ITEMS];
     uint32_t vmask, dr[ITEMS];  // digit << 16 | rank within the digit (< 2^14 keys per tile)
-    l1_windows<KeyT, NT, ITEMS>(rd, tile, s_raw, sm.ex, keys, vmask, err, [&]() {
+    uint32_t next_L = 0;        // read length of the next tile's job
+    const uint32_t nitem = item + nx;
+    l1_windows<KeyT, NT, ITEMS>(rd, tile, s_raw, sm.ex, s_mpos, keys, vmask, err, [&]() {
       // the records are decoded: the next tile's stream into the raw buffer
-      const uint32_t nitem = item + nx;
       if (nitem < hi) {
         if (!ONE) cursor_seek(cur, a.pre1, a.njobs, nitem, &s_job);
         const ReadsDesc rdn = ONE ? one.rd : job_desc(a, cur.job).rd;
         l1_prefetch_raw<NT, RAWW>(rdn, nitem - cur.lo, s_raw);
+        l1_patch_tail(rdn, nitem - cur.lo, s_raw);
+        next_L = rdn.L;
       }
     });
     // digit << 16 | rank of each slot; the digit is the top bits alone, or
@@ -1144,6 +1153,11 @@ __global__ __launch_bounds__(NT) void k_l1(BatchArgs a, JobDesc one) {
     for (int j = 0; j < ITEMS; ++j)
       if ((vmask >> j) & 1) sm.stage[dr[j]] = keys[j];
     __syncthreads();
+    if (nitem < hi) {  // the next tile's digit counts and record ends (this tile's are read)
+      s_h[tid] = 0;
+      if (tid == 0) s_h[1024] = 0;
+      if (tid < L1_MAXRT) s_mpos[tid] = next_L;
+    }
     write_staged<KeyT, NT, ITEMS>(sm.stage, out1 + (uint64_t)tile * m.tcap1, total, tid);
   }
 }

@@ -588,8 +588,10 @@ DK_DEV void scan_lds(uint32_t *s, uint32_t n, uint32_t *s_wave, uint32_t *s_tota
     pre = sum_waves_below<NW>(s_wave, wave, lane);
     tot = sum_waves_below<NW>(s_wave, NW, lane);
   } else {
+    // only the waves holding values have a non-zero total (4 * 64 per wave)
+    const uint32_t nwv = min(NW, (n + 255) / 256);
 #pragma nounroll
-    for (uint32_t w = 0; w < NW; ++w) {  // one read in flight: no registers held for all
+    for (uint32_t w = 0; w < nwv; ++w) {  // one read in flight: no registers held for all
       const uint32_t sw = s_wave[w];
       if (w < wave) pre += sw;
       tot += sw;

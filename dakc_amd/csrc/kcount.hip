@@ -693,7 +693,10 @@ DK_DEV uint32_t gather_segments(const KeyT *src, const uint32_t (&r0)[TPT], cons
   if constexpr (SHFL) {
     pre = sum_waves_below<NT / 64>(s_w64, tid >> 6, lane);
   } else {
-    for (uint32_t w = 0; w < (tid >> 6); ++w) pre += s_w64[w];
+    // waves past the segments (TPT per thread) hold none: their totals are 0
+    const uint32_t nwv = (nseg + 64 * TPT - 1) / (64 * TPT);
+    const uint32_t wb = (tid >> 6) < nwv ? (tid >> 6) : nwv;
+    for (uint32_t w = 0; w < wb; ++w) pre += s_w64[w];
   }
   if (tid == NT - 1) *s_total = (uint32_t)(pre + x);
   pre += x - (((uint64_t)cnt << 32) | sum);

@@ -1568,12 +1568,15 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
         if (hi - lo > 256) {
           big = true;
         } else {
+          // all 8 reads go out before one wait (a read under `t0 + q < hi`
+          // waited for each); past hi they see the next bins or, past H <=
+          // HT, the counts after the keys, and are masked out
           for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
             uint64_t y[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) y[q] = t0 + q < hi ? tk[t0 + q] : x[i];
+            for (int q = 0; q < 8; ++q) y[q] = tk[t0 + q];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) r += y[q] < x[i];
+            for (int q = 0; q < 8; ++q) r += (t0 + q < hi) & (y[q] < x[i]);
           }
         }
       }

@@ -1461,6 +1461,18 @@ DK_DEV void fin_clear(uint64_t *lds) {
 // pairs later take its place.
 // Returns false, having changed nothing the other path needs, when a key
 // finds no slot within 32 probes.  Uniform over the workgroup.
+// Slot after probe pr (0, 1, ...) of a probe sequence that steps 1, 2, 3, ...
+// from the home slot (offsets 0, 1, 3, 6, ...), wrapping at HT.  Growing steps
+// break up the runs of occupied slots that unit steps pile keys into, so
+// chains stay shorter at the same load; every copy of a key still walks the
+// same sequence.
+template <int HT>
+DK_DEV uint32_t probe_next(uint32_t h, int pr) {
+  static_assert(HT > 32, "a step is below HT");
+  const uint32_t n = h + (uint32_t)pr + 1;  // pr + 1 <= 32 < HT
+  return n >= (uint32_t)HT ? n - (uint32_t)HT : n;
+}
+
 template <int NT, int ITEMS, int HT>
 DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&keys)[ITEMS],
                          uint32_t mine, uint64_t *lds, uint32_t *s_h, uint32_t *s_wave,
@@ -1494,7 +1506,7 @@ DK_DEV bool fin_distinct(const JobDesc &J, const MsdArgs &m, const uint64_t (&ke
       uint32_t h = hh[u];
       uint64_t o = old[u];
       for (int pr = 0; pr < 32 && o != HASH_EMPTY && o != x; ++pr) {
-        h = h + 1 == (uint32_t)HT ? 0u : h + 1;
+        h = probe_next<HT>(h, pr);
         o = atomicCAS((unsigned long long *)&tk[h], HASH_EMPTY, x);
       }
       if (o == HASH_EMPTY || o == x) atomicAdd(&tc[h], 1u);
@@ -1651,7 +1663,7 @@ DK_DEV bool fin_distinct2(const JobDesc &J, const MsdArgs &m, const K2 (&keys)[I
       uint32_t h = hh[u];
       uint64_t o = old[u];
       for (int pr = 0; pr < 32 && o != HASH_EMPTY && o != c; ++pr) {
-        h = h + 1 == (uint32_t)HT ? 0u : h + 1;
+        h = probe_next<HT>(h, pr);
         o = atomicCAS((unsigned long long *)&kc[h], HASH_EMPTY, c);
       }
       if (o == HASH_EMPTY) khi[h] = keys[u].hi;  // claimed here

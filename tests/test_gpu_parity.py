@@ -256,6 +256,24 @@ def test_repeat_runs_identical(kc):
         assert np.array_equal(a.keys, b.keys) and np.array_equal(a.counts, b.counts)
 
 
+def test_final_tables_near_full():
+    """Reads with 2% substitutions at 10x coverage: 58% of the k-mers are
+    distinct, so a final item of about 13,700 keys fills its 10,240-slot hash
+    table to about 78% (long probe chains, some wrapping past the last slot,
+    and items whose chains run out take the general path).  The count stays
+    on the MSD path and matches the oracle."""
+    p = _lib.gen_params(11, 200_000, 150, genome_len=3_000_000, sub_rate=0.02)
+    data = _lib.generate_reads(p)
+    with KmerCounter(k=31, read_len=150) as kc2:
+        for canonical in (False, True):
+            g = kc2.count(data, canonical=canonical)
+            st = kc2.last_stats()
+            assert st.path == 2 and st.msd_reruns == 0
+            ora = oracle.count(data, 150, 31, canonical)
+            assert_same(g, ora)
+            assert 0.5 < len(ora[1]) / ora[2] < 0.7  # the load the test is for
+
+
 def test_items_larger_than_lds_go_digit_by_digit(tmp_path):
     """A tuned fill above 100% packs more second-level digits into a final
     item than LDS holds: every item is then counted digit by digit in its
